@@ -1,0 +1,195 @@
+/*
+ * pysit_amd.h -- C ABI of the MI355X (gfx950) library behind the drop-in
+ * constant-density acoustic time solver.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success and a negative code on error;
+ *     pysit_amd_last_error() then returns a message (thread-local).  No C++
+ *     exception crosses the ABI.
+ *   - Array arguments are DEVICE pointers unless stated otherwise.  Grids are
+ *     the reference's padded grids with the reference's flat layout:
+ *     2D idx = i*nz + k, 3D idx = (i*ny + j)*nz + k (z contiguous).
+ *   - `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ *     Calls on one stream are ordered; a batch descriptor must not be used
+ *     from two threads at once.
+ *   - dtype codes: 0 = float32, 1 = float64.
+ */
+#ifndef PYSIT_AMD_H
+#define PYSIT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYSIT_AMD_OK 0
+#define PYSIT_AMD_EINVAL (-1)   /* bad argument (shape, order, dtype, pointer) */
+#define PYSIT_AMD_EDEVICE (-2)  /* a HIP runtime call failed */
+
+/* Message of the last error on this thread ("" if none). */
+const char *pysit_amd_last_error(void);
+
+/* Library version string. */
+const char *pysit_amd_version(void);
+
+/* ABI self-check for bindings: returns 0 when the caller's sizes of
+ * pysit_amd_shot and pysit_amd_batch match the library's, else -1. */
+int pysit_amd_abi_check(int shot_size, int batch_size);
+
+/* ------------------------------------------------------------------------ */
+/* Step entry points: one leapfrog+PML step with the reference kernel        */
+/* contract.  Replace the SWIG-exported                                      */
+/*   constant_density_acoustic_time_scalar_2D_{2,4,6,8}os                     */
+/* (pysit/solvers/constant_density_acoustic/time/scalar/                     */
+/*  constant_density_acoustic_time_scalar.i:57-84, C++ signature             */
+/*  constant_density_acoustic_time_scalar_2D.h:44-62) with the same argument */
+/* order minus the (rows, cols) pairs: inputs km1_u, k_Phix, k_Phiz, k_u, C, */
+/* rhs (padded nx*nz), PML arrays with lengths, dt, dx, dz, nx, nz, and the  */
+/* caller-owned outputs kp1_Phix, kp1_Phiz, kp1_u.  Outputs must not alias  */
+/* inputs.  order in {2,4,6,8}.                                              */
+/* ------------------------------------------------------------------------ */
+int pysit_amd_cda_step2d_f32(int order, const float *km1_u, const float *k_phix,
+                             const float *k_phiz, const float *k_u, const float *C,
+                             const float *rhs, const float *xlpml, int n_xlpml,
+                             const float *xrpml, int n_xrpml, const float *zlpml,
+                             int n_zlpml, const float *zrpml, int n_zrpml, double dt,
+                             double dx, double dz, int nx, int nz, float *kp1_phix,
+                             float *kp1_phiz, float *kp1_u, void *stream);
+int pysit_amd_cda_step2d_f64(int order, const double *km1_u, const double *k_phix,
+                             const double *k_phiz, const double *k_u, const double *C,
+                             const double *rhs, const double *xlpml, int n_xlpml,
+                             const double *xrpml, int n_xrpml, const double *zlpml,
+                             int n_zlpml, const double *zrpml, int n_zrpml, double dt,
+                             double dx, double dz, int nx, int nz, double *kp1_phix,
+                             double *kp1_phiz, double *kp1_u, void *stream);
+
+/* 3D counterparts of constant_density_acoustic_time_scalar_3D_{2,4,6,8}os
+ * (constant_density_acoustic_time_scalar.i:78-84, C++ signature
+ * constant_density_acoustic_time_scalar_3D.h:47-72). */
+int pysit_amd_cda_step3d_f32(int order, const float *km1_u, const float *k_phix,
+                             const float *k_phiy, const float *k_phiz, const float *k_psi,
+                             const float *k_u, const float *C, const float *rhs,
+                             const float *xlpml, int n_xlpml, const float *xrpml,
+                             int n_xrpml, const float *ylpml, int n_ylpml,
+                             const float *yrpml, int n_yrpml, const float *zlpml,
+                             int n_zlpml, const float *zrpml, int n_zrpml, double dt,
+                             double dx, double dy, double dz, int nx, int ny, int nz,
+                             float *kp1_phix, float *kp1_phiy, float *kp1_phiz,
+                             float *kp1_psi, float *kp1_u, void *stream);
+int pysit_amd_cda_step3d_f64(int order, const double *km1_u, const double *k_phix,
+                             const double *k_phiy, const double *k_phiz,
+                             const double *k_psi, const double *k_u, const double *C,
+                             const double *rhs, const double *xlpml, int n_xlpml,
+                             const double *xrpml, int n_xrpml, const double *ylpml,
+                             int n_ylpml, const double *yrpml, int n_yrpml,
+                             const double *zlpml, int n_zlpml, const double *zrpml,
+                             int n_zrpml, double dt, double dx, double dy, double dz,
+                             int nx, int ny, int nz, double *kp1_phix, double *kp1_phiy,
+                             double *kp1_phiz, double *kp1_psi, double *kp1_u,
+                             void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Batched shot loops.  Replace the per-step Python loops of                */
+/* TemporalModeling.forward_model (pysit/modeling/temporal_modeling.py:      */
+/* 60-169) and adjoint_model (:228-387) for many shots at once: one kernel   */
+/* launch advances every active shot of the batch by one time step, with    */
+/* source injection, receiver sampling, the dWaveOp store (forward) and the */
+/* FP64 imaging condition (adjoint) fused into the step.                    */
+/*                                                                           */
+/* A batch is described by device tables the host builds once per geometry  */
+/* (see pysit_amd/device.py, which fills them):                              */
+/* ------------------------------------------------------------------------ */
+
+/* Per-shot descriptor (device array, one per shot). */
+typedef struct {
+    int nx, ny, nz;        /* padded grid of this shot (ny = 1 in 2D)            */
+    int nt;                /* number of time steps                               */
+    int kshift;            /* local step k = slot - kshift                       */
+    int ntx, nty, ntz;     /* tiles of this shot along x, y, z                   */
+    long long off;         /* element offset of this shot in the field arrays    */
+    /* model (wave speed, dtype of the batch): value at padded (i,j,k) is
+     * model[((clamp(i-mx_off,mx_lo,mx_hi))*mny + clamp(j-my_off,..))*mnz
+     *        + clamp(k-mz_off,mz_lo,mz_hi)]  -- edge padding by clamping,
+     * the 'edge' pad of solver_base.py:107-117.                              */
+    const void *model;
+    int mny, mnz;
+    int mx_off, mx_lo, mx_hi, my_off, my_lo, my_hi, mz_off, mz_lo, mz_hi;
+    /* PML arrays in the reference form (dtype of the batch) with lengths    */
+    const void *xl, *xr, *yl, *yr, *zl, *zr;
+    int nxl, nxr, nyl, nyr, nzl, nzr;
+    /* injection amplitudes: amp[k * namp + chan] (FP64)                     */
+    const double *amp;
+    int namp;
+    int nrec;              /* receivers of this shot                             */
+    double *trace;         /* trace[k * nrec + r] (FP64), forward sampling       */
+} pysit_amd_shot;
+
+typedef struct {
+    int dtype;             /* 0 f32, 1 f64                                       */
+    int dim;               /* 2 or 3                                             */
+    int order;             /* 2, 4, 6, 8                                         */
+    int tile_x;            /* rows along x per wave tile                         */
+    int tile_y;            /* rows along y per wave tile (3D)                    */
+    double dt, dx, dy, dz;
+    int n_shots;
+    const pysit_amd_shot *shots;
+    /* tiles: int4 {shot, tile index within the shot, flags, 0}; flags bit0 =
+     * the tile is within `order/2` nodes of a PML node (full path).  Shots
+     * sorted so that, for every slot, the active tiles form a prefix.        */
+    int n_tiles;
+    const int *tiles;
+    /* injection entries per tile, CSR: entries [inj_ptr[t], inj_ptr[t+1])
+     * of tile t, sorted by x (then y); inj_pos = {x, y, z, chan}           */
+    const int *inj_ptr;
+    const int *inj_pos;
+    const double *inj_w;
+    /* receiver sampling: rec {shot, r, e0, e1}; entries rec_idx (flat index
+     * in the shot's padded grid) and rec_w (FP64, weight * prod(deltas))    */
+    int n_rec;
+    const int *recs;
+    const long long *rec_idx;
+    const double *rec_w;
+    /* state: each field has two buffers; the local step parity picks which
+     * holds time level k.  2D uses u, phix, phiz; 3D adds phiy and psi.     */
+    void *u[2], *phix[2], *phiy[2], *phiz[2], *psi[2];
+    long long total_n;     /* sum of shot grid sizes                             */
+    /* dWaveOp store (dtype of the batch): slot-major, dw[(slot - seg_base) *
+     * total_n + off + idx]; NULL = not stored                               */
+    void *dw;
+    /* imaging-condition accumulator (FP64, total_n), adjoint only           */
+    double *ic;
+    int imaging_period;
+} pysit_amd_batch;
+
+/* Run slots [slot_begin, slot_end) forward (direction = +1) or
+ * (slot_end-1 .. slot_begin) backward (direction = -1).
+ *   mode 0: propagate only (+ receiver sampling when recs are given)
+ *   mode 1: propagate and store dWaveOp for k % imaging_period == 0
+ *   mode 2: adjoint: ic += q_k * dWaveOp_k (k % imaging_period == 0), then
+ *           propagate; receiver sampling is skipped
+ * seg_base: slot index of the dWaveOp store's first slot.
+ * active_tiles[j], active_recs[j] (HOST arrays indexed by slot, may be NULL =
+ * all): how many leading tiles / receivers are active at slot j.
+ * A backward run starts with q at the state the previous call left (both
+ * zero for a fresh accumulation). */
+int pysit_amd_batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int slot_end,
+                        int direction, int seg_base, const int *active_tiles,
+                        const int *active_recs, void *stream);
+
+/* Fold (ModelPerturbationBase.add_padding, pysit/solvers/model_parameter.py:
+ * 537-593) of one padded FP64 image onto its unpadded grid: the PML slabs are
+ * summed onto the first/last physical plane of each axis (x, then y, then z),
+ * the result is scaled and written (accumulate = 0) or added (accumulate = 1)
+ * into `out`, which points at the window origin inside a larger array whose
+ * row strides are out_ny * out_nz (x) and out_nz (y).  nx, ny, nz are the
+ * padded sizes, lx/rx/... the padding widths (2D: ny = 1, ly = ry = 0). */
+int pysit_amd_fold(int dim, int nx, int ny, int nz, int lx, int rx, int ly, int ry,
+                   int lz, int rz, const double *in, double scale, double *out,
+                   int out_ny, int out_nz, int accumulate, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYSIT_AMD_H */

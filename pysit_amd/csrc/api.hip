@@ -1,0 +1,237 @@
+// api.hip -- the extern "C" surface of libpysit_amd (declared in
+// include/pysit_amd.h): argument checks, dispatch on dimension and dtype,
+// error strings, and the PML fold kernel.
+#include "common.h"
+
+namespace pa {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+void clear_error() { g_err.clear(); }
+
+template <typename T>
+int run2d(const pysit_amd_batch &, int, int, int, int, int, const int *, const int *, hipStream_t);
+template <typename T>
+int run3d(const pysit_amd_batch &, int, int, int, int, int, const int *, const int *, hipStream_t);
+
+template <typename T>
+int step2d_entry(int, const T *, const T *, const T *, const T *, const T *, const T *, const T *,
+                 int, const T *, int, const T *, int, const T *, int, double, double, double, int,
+                 int, T *, T *, T *, hipStream_t);
+template <typename T>
+int step3d_entry(int, const T *, const T *, const T *, const T *, const T *, const T *, const T *,
+                 const T *, const T *, int, const T *, int, const T *, int, const T *, int,
+                 const T *, int, const T *, int, double, double, double, double, int, int, int, T *,
+                 T *, T *, T *, T *, hipStream_t);
+
+// ---- fold (add_padding) ---------------------------------------------------
+// One thread per unpadded output node; the PML slabs that fold onto it are
+// summed x first, then y, then z, as the reference's successive passes do.
+__global__ void k_fold(int nx, int ny, int nz, int lx, int rx, int ly, int ry, int lz, int rz,
+                       const double *__restrict__ in, double scale, double *__restrict__ out,
+                       int out_ny, int out_nz, int accumulate)
+{
+    const int ox = nx - lx - rx, oy = ny - ly - ry, oz = nz - lz - rz;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)ox * oy * oz) return;
+    const int ko = (int)(t % oz);
+    const int jo = (int)((t / oz) % oy);
+    const int io = (int)(t / ((long long)oz * oy));
+    auto A = [&](int i, int j, int k) { return in[((long long)i * ny + j) * nz + k]; };
+    auto X = [&](int j, int k) {
+        double v = A(io + lx, j, k);
+        if (io == 0) {
+            double s = 0.0;
+            for (int i = 0; i < lx; ++i) s += A(i, j, k);
+            v = v + s;
+        }
+        if (io == ox - 1) {
+            double s = 0.0;
+            for (int i = lx + ox; i < nx; ++i) s += A(i, j, k);
+            v = v + s;
+        }
+        return v;
+    };
+    auto Y = [&](int k) {
+        double v = X(jo + ly, k);
+        if (jo == 0) {
+            double s = 0.0;
+            for (int j = 0; j < ly; ++j) s += X(j, k);
+            v = v + s;
+        }
+        if (jo == oy - 1) {
+            double s = 0.0;
+            for (int j = ly + oy; j < ny; ++j) s += X(j, k);
+            v = v + s;
+        }
+        return v;
+    };
+    double v = Y(ko + lz);
+    if (ko == 0) {
+        double s = 0.0;
+        for (int k = 0; k < lz; ++k) s += Y(k);
+        v = v + s;
+    }
+    if (ko == oz - 1) {
+        double s = 0.0;
+        for (int k = lz + oz; k < nz; ++k) s += Y(k);
+        v = v + s;
+    }
+    double *o = out + ((long long)io * out_ny + jo) * out_nz + ko;
+    *o = accumulate ? *o + scale * v : scale * v;
+}
+
+}  // namespace pa
+
+using namespace pa;
+
+extern "C" {
+
+const char *pysit_amd_last_error(void) { return g_err.c_str(); }
+
+const char *pysit_amd_version(void) { return "pysit_amd 0.1 (gfx950)"; }
+
+int pysit_amd_abi_check(int shot_size, int batch_size)
+{
+    clear_error();
+    if (shot_size != (int)sizeof(pysit_amd_shot) || batch_size != (int)sizeof(pysit_amd_batch)) {
+        set_error("struct size mismatch: library has shot " + std::to_string(sizeof(pysit_amd_shot)) +
+                  ", batch " + std::to_string(sizeof(pysit_amd_batch)));
+        return PYSIT_AMD_EINVAL;
+    }
+    return PYSIT_AMD_OK;
+}
+
+int pysit_amd_cda_step2d_f32(int order, const float *km1_u, const float *k_phix,
+                             const float *k_phiz, const float *k_u, const float *C,
+                             const float *rhs, const float *xlpml, int n_xlpml,
+                             const float *xrpml, int n_xrpml, const float *zlpml, int n_zlpml,
+                             const float *zrpml, int n_zrpml, double dt, double dx, double dz,
+                             int nx, int nz, float *kp1_phix, float *kp1_phiz, float *kp1_u,
+                             void *stream)
+{
+    clear_error();
+    return step2d_entry<float>(order, km1_u, k_phix, k_phiz, k_u, C, rhs, xlpml, n_xlpml, xrpml,
+                               n_xrpml, zlpml, n_zlpml, zrpml, n_zrpml, dt, dx, dz, nx, nz,
+                               kp1_phix, kp1_phiz, kp1_u, (hipStream_t)stream);
+}
+
+int pysit_amd_cda_step2d_f64(int order, const double *km1_u, const double *k_phix,
+                             const double *k_phiz, const double *k_u, const double *C,
+                             const double *rhs, const double *xlpml, int n_xlpml,
+                             const double *xrpml, int n_xrpml, const double *zlpml, int n_zlpml,
+                             const double *zrpml, int n_zrpml, double dt, double dx, double dz,
+                             int nx, int nz, double *kp1_phix, double *kp1_phiz, double *kp1_u,
+                             void *stream)
+{
+    clear_error();
+    return step2d_entry<double>(order, km1_u, k_phix, k_phiz, k_u, C, rhs, xlpml, n_xlpml, xrpml,
+                                n_xrpml, zlpml, n_zlpml, zrpml, n_zrpml, dt, dx, dz, nx, nz,
+                                kp1_phix, kp1_phiz, kp1_u, (hipStream_t)stream);
+}
+
+int pysit_amd_cda_step3d_f32(int order, const float *km1_u, const float *k_phix,
+                             const float *k_phiy, const float *k_phiz, const float *k_psi,
+                             const float *k_u, const float *C, const float *rhs,
+                             const float *xlpml, int n_xlpml, const float *xrpml, int n_xrpml,
+                             const float *ylpml, int n_ylpml, const float *yrpml, int n_yrpml,
+                             const float *zlpml, int n_zlpml, const float *zrpml, int n_zrpml,
+                             double dt, double dx, double dy, double dz, int nx, int ny, int nz,
+                             float *kp1_phix, float *kp1_phiy, float *kp1_phiz, float *kp1_psi,
+                             float *kp1_u, void *stream)
+{
+    clear_error();
+    return step3d_entry<float>(order, km1_u, k_phix, k_phiy, k_phiz, k_psi, k_u, C, rhs, xlpml,
+                               n_xlpml, xrpml, n_xrpml, ylpml, n_ylpml, yrpml, n_yrpml, zlpml,
+                               n_zlpml, zrpml, n_zrpml, dt, dx, dy, dz, nx, ny, nz, kp1_phix,
+                               kp1_phiy, kp1_phiz, kp1_psi, kp1_u, (hipStream_t)stream);
+}
+
+int pysit_amd_cda_step3d_f64(int order, const double *km1_u, const double *k_phix,
+                             const double *k_phiy, const double *k_phiz, const double *k_psi,
+                             const double *k_u, const double *C, const double *rhs,
+                             const double *xlpml, int n_xlpml, const double *xrpml, int n_xrpml,
+                             const double *ylpml, int n_ylpml, const double *yrpml, int n_yrpml,
+                             const double *zlpml, int n_zlpml, const double *zrpml, int n_zrpml,
+                             double dt, double dx, double dy, double dz, int nx, int ny, int nz,
+                             double *kp1_phix, double *kp1_phiy, double *kp1_phiz,
+                             double *kp1_psi, double *kp1_u, void *stream)
+{
+    clear_error();
+    return step3d_entry<double>(order, km1_u, k_phix, k_phiy, k_phiz, k_psi, k_u, C, rhs, xlpml,
+                                n_xlpml, xrpml, n_xrpml, ylpml, n_ylpml, yrpml, n_yrpml, zlpml,
+                                n_zlpml, zrpml, n_zrpml, dt, dx, dy, dz, nx, ny, nz, kp1_phix,
+                                kp1_phiy, kp1_phiz, kp1_psi, kp1_u, (hipStream_t)stream);
+}
+
+int pysit_amd_batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int slot_end,
+                        int direction, int seg_base, const int *active_tiles,
+                        const int *active_recs, void *stream)
+{
+    clear_error();
+    PA_CHECK(b != nullptr, "null batch");
+    const pysit_amd_batch &B = *b;
+    PA_CHECK(B.dim == 2 || B.dim == 3, "dim must be 2 or 3");
+    PA_CHECK(B.dtype == 0 || B.dtype == 1, "dtype must be 0 (f32) or 1 (f64)");
+    PA_CHECK(B.order == 2 || B.order == 4 || B.order == 6 || B.order == 8,
+             "order must be 2, 4, 6 or 8");
+    PA_CHECK(mode >= 0 && mode <= 2, "mode must be 0, 1 or 2");
+    PA_CHECK(direction == 1 || direction == -1, "direction must be +1 or -1");
+    PA_CHECK(slot_begin >= 0 && slot_end >= slot_begin, "bad slot range");
+    PA_CHECK(B.tile_x >= 1 && (B.dim == 2 || B.tile_y >= 1), "tile sizes must be positive");
+    PA_CHECK(B.n_shots >= 1 && B.shots && B.n_tiles >= 0 && (B.n_tiles == 0 || B.tiles),
+             "shot or tile table missing");
+    PA_CHECK(B.u[0] && B.u[1] && B.phix[0] && B.phix[1] && B.phiz[0] && B.phiz[1],
+             "state buffers missing");
+    PA_CHECK(B.dim == 2 || (B.phiy[0] && B.phiy[1] && B.psi[0] && B.psi[1]),
+             "3D state buffers missing");
+    PA_CHECK(B.imaging_period >= 1, "imaging_period must be >= 1");
+    PA_CHECK(mode != 2 || B.ic, "adjoint mode needs the imaging accumulator");
+    PA_CHECK(mode == 0 || B.dw, "modes 1 and 2 need the dWaveOp store");
+    PA_CHECK(B.n_rec == 0 || (B.recs && B.rec_idx && B.rec_w), "receiver tables missing");
+    PA_CHECK(seg_base >= 0 && seg_base <= slot_begin, "seg_base must not exceed slot_begin");
+    if (active_tiles || active_recs) {
+        for (int j = slot_begin; j < slot_end; ++j) {
+            PA_CHECK(!active_tiles || (active_tiles[j] >= 0 && active_tiles[j] <= B.n_tiles),
+                     "active tile count out of range");
+            PA_CHECK(mode == 2 || !active_recs || (active_recs[j] >= 0 && active_recs[j] <= B.n_rec),
+                     "active receiver count out of range");
+        }
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (B.dim == 2)
+        return B.dtype ? run2d<double>(B, mode, slot_begin, slot_end, direction, seg_base,
+                                       active_tiles, active_recs, st)
+                       : run2d<float>(B, mode, slot_begin, slot_end, direction, seg_base,
+                                      active_tiles, active_recs, st);
+    return B.dtype ? run3d<double>(B, mode, slot_begin, slot_end, direction, seg_base,
+                                   active_tiles, active_recs, st)
+                   : run3d<float>(B, mode, slot_begin, slot_end, direction, seg_base,
+                                  active_tiles, active_recs, st);
+}
+
+int pysit_amd_fold(int dim, int nx, int ny, int nz, int lx, int rx, int ly, int ry, int lz, int rz,
+                   const double *in, double scale, double *out, int out_ny, int out_nz,
+                   int accumulate, void *stream)
+{
+    clear_error();
+    PA_CHECK(dim == 2 || dim == 3, "dim must be 2 or 3");
+    if (dim == 2) {
+        ny = 1;
+        ly = ry = 0;
+        out_ny = 1;
+    }
+    PA_CHECK(in && out, "null array argument");
+    PA_CHECK(lx >= 0 && rx >= 0 && ly >= 0 && ry >= 0 && lz >= 0 && rz >= 0, "negative padding");
+    const int ox = nx - lx - rx, oy = ny - ly - ry, oz = nz - lz - rz;
+    PA_CHECK(ox >= 1 && oy >= 1 && oz >= 1, "padding leaves an empty grid");
+    PA_CHECK(out_nz >= oz && out_ny >= oy, "output strides smaller than the unpadded block");
+    const long long n = (long long)ox * oy * oz;
+    const int blocks = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(k_fold, dim3(blocks), dim3(256), 0, (hipStream_t)stream, nx, ny, nz, lx,
+                       rx, ly, ry, lz, rz, in, scale, out, out_ny, out_nz, accumulate);
+    PA_HIP(hipGetLastError());
+    return PYSIT_AMD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,128 @@
+// common.h -- shared device helpers and host-side error plumbing for the
+// pysit_amd HIP library (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "../../include/pysit_amd.h"
+
+namespace pa {
+
+// ---- errors (thread-local message behind pysit_amd_last_error) ----------
+void set_error(const std::string &msg);
+void clear_error();
+
+#define PA_HIP(call)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            ::pa::set_error(std::string(#call) + ": " + hipGetErrorString(e_));   \
+            return PYSIT_AMD_EDEVICE;                                             \
+        }                                                                         \
+    } while (0)
+
+#define PA_CHECK(cond, msg)                                                       \
+    do {                                                                          \
+        if (!(cond)) {                                                            \
+            ::pa::set_error(msg);                                                 \
+            return PYSIT_AMD_EINVAL;                                              \
+        }                                                                         \
+    } while (0)
+
+// ---- centred finite-difference weights, accuracy 2r (r = 1..4) ----------
+// first derivative at offset +j (offset -j has the opposite sign)
+__host__ __device__ constexpr double fd1(int r, int j)
+{
+    return r == 1 ? (j == 1 ? 1. / 2. : 0.)
+         : r == 2 ? (j == 1 ? 2. / 3. : j == 2 ? -1. / 12. : 0.)
+         : r == 3 ? (j == 1 ? 3. / 4. : j == 2 ? -3. / 20. : j == 3 ? 1. / 60. : 0.)
+         : (j == 1 ? 4. / 5. : j == 2 ? -1. / 5. : j == 3 ? 4. / 105. : j == 4 ? -1. / 280. : 0.);
+}
+// second derivative at offset +-j (j = 0 is the centre)
+__host__ __device__ constexpr double fd2(int r, int j)
+{
+    return r == 1 ? (j == 0 ? -2.0 : j == 1 ? 1.0 : 0.)
+         : r == 2 ? (j == 0 ? -5. / 2. : j == 1 ? 4. / 3. : j == 2 ? -1. / 12. : 0.)
+         : r == 3 ? (j == 0 ? -49. / 18. : j == 1 ? 3. / 2. : j == 2 ? -3. / 20. : j == 3 ? 1. / 90. : 0.)
+         : (j == 0 ? -205. / 72. : j == 1 ? 8. / 5. : j == 2 ? -1. / 5. : j == 3 ? 8. / 315. : j == 4 ? -1. / 560. : 0.);
+}
+
+// Per-axis stencil coefficients with the grid spacing folded in.
+template <typename T>
+struct AxisCoef {
+    T d1[4];  // fd1(r, j+1) / h
+    T d2[5];  // fd2(r, j) / h^2
+};
+
+template <typename T>
+inline AxisCoef<T> axis_coef(int r, double h)
+{
+    AxisCoef<T> c{};
+    for (int j = 0; j < 4; ++j) c.d1[j] = (T)(fd1(r, j + 1) / h);
+    for (int j = 0; j < 5; ++j) c.d2[j] = (T)(fd2(r, j) / (h * h));
+    return c;
+}
+
+template <typename T>
+struct TimeCoef {
+    T dt, dt2, inv_dt2, inv_2dt, two_dt2;
+};
+
+template <typename T>
+inline TimeCoef<T> time_coef(double dt)
+{
+    TimeCoef<T> c;
+    c.dt = (T)dt;
+    c.dt2 = (T)(dt * dt);
+    c.inv_dt2 = (T)(1.0 / (dt * dt));
+    c.inv_2dt = (T)(1.0 / (2.0 * dt));
+    c.two_dt2 = (T)(2.0 * dt * dt);
+    return c;
+}
+
+// Block index remap so that the blocks the dispatcher sends to one XCD
+// (round robin over 8 XCDs) cover one contiguous range of logical blocks:
+// neighbouring tiles (and the shots of one tile) then share that XCD's L2.
+__device__ __forceinline__ int xcd_block(int b, int nb)
+{
+    constexpr int NXCD = 8;
+    const int xcd = b % NXCD, q = b / NXCD;
+    const int full = nb / NXCD, rem = nb % NXCD;
+    return (xcd < rem) ? xcd * (full + 1) + q : rem * (full + 1) + (xcd - rem) * full + q;
+}
+
+// LDS exchange inside one wavefront: order the LDS stores of all lanes before
+// the loads that follow (compiler barrier plus wavefront-scope fences).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Kernel-side PML profile value at node i of an axis of n nodes (reference
+// indexing: left array by node, right array offset by one towards the edge;
+// the edge node itself is never updated, callers pass i <= n-2).
+template <typename T>
+__device__ __forceinline__ T sigma_kernel(const T *l, int nl, const T *r, int nr, int i, int n)
+{
+    if (nl > 0 && i < nl) return l[i];
+    if (nr > 0 && i >= n - nr && i <= n - 2) return r[nr - ((n - 1) - i)];
+    return T(0);
+}
+// Mesh PML profile (StructuredPML.eval_on_mesh + build_sigma): the right
+// array is laid on the last nr nodes without the shift.
+template <typename T>
+__device__ __forceinline__ T sigma_mesh(const T *l, int nl, const T *r, int nr, int i, int n)
+{
+    if (nl > 0 && i < nl) return l[i];
+    if (nr > 0 && i >= n - nr) return r[i - (n - nr)];
+    return T(0);
+}
+
+}  // namespace pa

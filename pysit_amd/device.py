@@ -1,0 +1,540 @@
+"""Device-side shot batches: tables, state, and the fused time loops.
+
+A :class:`ShotGeometry` describes one propagation in the reference's terms
+(padded grid, the kernel PML arrays, the edge-padded model window, point
+operators as sparse entry lists, the wavelet amplitudes).  A
+:class:`DeviceBatch` packs any number of them for one launch per time step of
+``pysit_amd_batch_run`` and runs
+
+* ``forward``      -- record traces (and optionally keep dWaveOp),
+* ``migrate``      -- the time-reversed adjoint with the FP64 imaging
+  condition, recomputing dWaveOp from state checkpoints when the store does
+  not fit the memory budget,
+* ``fold``         -- the PML fold of each shot's image onto the model grid.
+
+Time slots: all shots of a batch end on the same slot; shot ``s`` with
+``nt_s`` steps starts at slot ``kshift_s = max_nt - nt_s``.  Shots are sorted
+by ``kshift`` so that the active tiles (and receivers) of every slot form a
+prefix of the tables and each launch covers exactly the active work.
+
+Device memory comes from torch (plumbing only); every kernel is ours.
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _native as N
+
+TILE_Z = 64      # z nodes per wavefront (the contiguous axis)
+TILE_Y3 = 8      # y rows per 3D workgroup
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        raise N.NativeError("no ROCm device visible: the pysit_amd solver path runs on the GPU only")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class ShotGeometry:
+    """One propagation, in padded-grid terms.
+
+    Parameters
+    ----------
+    shape : padded grid ``(nx, nz)`` or ``(nx, ny, nz)``
+    kpml : per axis ``(left, right)`` kernel PML arrays (the reference's
+        ``mesh.<axis>.{lbc,rbc}.sigma``; empty arrays for non-PML sides)
+    window : per axis ``(off, lo, hi)``; the model value of padded node ``i``
+        is ``model[clamp(i - off, lo, hi)]`` (edge padding by clamping)
+    nt : number of time steps
+    src : ``(pos, chan, w)`` injection entries, ``pos`` (n, dim) padded
+        integer coordinates, in the reference's accumulation order
+    amp : ``(nt, n_src_chan)`` FP64 amplitudes, ``amp[k, c] = intensity*w(k dt)``
+    rec : ``(idx, chan, w)`` sampling entries: flat padded index, receiver,
+        weight (already times prod(deltas)), grouped by receiver
+    nrec : receivers
+    adj : ``(pos, chan, w)`` adjoint injection entries (weights without
+        prod(deltas)), ``chan`` = receiver
+    """
+
+    def __init__(self, shape, kpml, window, nt, src, amp, rec, nrec, adj=None):
+        self.shape = tuple(int(s) for s in shape)
+        self.dim = len(self.shape)
+        self.kpml = [(np.asarray(l, dtype=np.float64).reshape(-1), np.asarray(r, dtype=np.float64).reshape(-1))
+                     for l, r in kpml]
+        self.window = [tuple(int(v) for v in w) for w in window]
+        self.nt = int(nt)
+        self.src = _as_entries(src, self.dim)
+        self.amp = np.ascontiguousarray(amp, dtype=np.float64)
+        if self.amp.ndim != 2 or self.amp.shape[0] < self.nt:
+            raise ValueError("amp must be (>= nt, n_src_chan)")
+        self.rec = (np.asarray(rec[0], dtype=np.int64), np.asarray(rec[1], dtype=np.int64),
+                    np.asarray(rec[2], dtype=np.float64))
+        self.nrec = int(nrec)
+        self.adj = _as_entries(adj, self.dim) if adj is not None else None
+        if len(self.kpml) != self.dim or len(self.window) != self.dim:
+            raise ValueError("kpml and window need one entry per axis")
+
+    @property
+    def npts(self):
+        return int(np.prod(self.shape))
+
+
+def _as_entries(e, dim):
+    pos, chan, w = e
+    pos = np.asarray(pos, dtype=np.int64).reshape(-1, dim)
+    return pos, np.asarray(chan, dtype=np.int64).reshape(-1), np.asarray(w, dtype=np.float64).reshape(-1)
+
+
+class DeviceBatch:
+    """Tables and state of a shot batch on one GPU."""
+
+    def __init__(self, shots, model, deltas, dt, order, dtype=np.float32, tile_x=None,
+                 imaging_period=1, memory_budget=None):
+        if not shots:
+            raise ValueError("empty shot batch")
+        self.dev = _dev()
+        self.dim = shots[0].dim
+        if any(s.dim != self.dim for s in shots):
+            raise ValueError("all shots of a batch share the dimension")
+        if order not in (2, 4, 6, 8):
+            raise ValueError("spatial_accuracy_order must be 2, 4, 6 or 8")
+        self.order = int(order)
+        self.R = self.order // 2
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in (np.float32, np.float64):
+            raise ValueError("dtype must be float32 or float64")
+        self.tdtype = torch.float32 if self.dtype == np.float32 else torch.float64
+        self.deltas = tuple(float(d) for d in deltas)
+        self.dt = float(dt)
+        self.imaging_period = int(imaging_period)
+        self.tile_x = int(tile_x) if tile_x else (16 if self.dim == 2 else 8)
+        self.memory_budget = memory_budget
+
+        model = np.ascontiguousarray(model, dtype=self.dtype)
+        if model.ndim != self.dim:
+            raise ValueError("model must have the grid's dimension")
+        self.model_shape = model.shape
+        self.model = torch.from_numpy(model).to(self.dev)
+
+        self.max_nt = max(s.nt for s in shots)
+        kshift = [self.max_nt - s.nt for s in shots]
+        self.order_idx = sorted(range(len(shots)), key=lambda i: kshift[i])  # stable
+        self.shots = [shots[i] for i in self.order_idx]
+        self.kshift = [kshift[i] for i in self.order_idx]
+        self.n_shots = len(self.shots)
+        self._build_tables()
+        self._alloc_state()
+
+    # ------------------------------------------------------------------ tables
+    def _tiles_of(self, s):
+        """(ntx, nty, ntz) and the per-tile PML flag array (tloc order)."""
+        R = self.R
+        if self.dim == 2:
+            nx, nz = s.shape
+            ntz = -(-nz // TILE_Z)
+            ntx = -(-nx // self.tile_x)
+            flags = np.zeros(ntx * ntz, dtype=np.int32)
+            (xl, xr), (zl, zr) = s.kpml
+            for tx in range(ntx):
+                x0, x1 = tx * self.tile_x, min(nx, (tx + 1) * self.tile_x) - 1
+                xp = (x0 - R < xl.size) or (x1 + R >= nx - xr.size)
+                for tz in range(ntz):
+                    z0, z1 = tz * TILE_Z, min(nz, (tz + 1) * TILE_Z) - 1
+                    zp = (z0 - R < zl.size) or (z1 + R >= nz - zr.size)
+                    flags[tx * ntz + tz] = 1 if (xp or zp) else 0
+            return (ntx, 1, ntz), flags
+        nx, ny, nz = s.shape
+        ntz = -(-nz // TILE_Z)
+        nty = -(-ny // TILE_Y3)
+        ntx = -(-nx // self.tile_x)
+        (xl, xr), (yl, yr), (zl, zr) = s.kpml
+        flags = np.zeros(ntx * nty * ntz, dtype=np.int32)
+        for tx in range(ntx):
+            x0, x1 = tx * self.tile_x, min(nx, (tx + 1) * self.tile_x) - 1
+            xp = (x0 - R < xl.size) or (x1 + R >= nx - xr.size)
+            for ty in range(nty):
+                y0, y1 = ty * TILE_Y3, min(ny, (ty + 1) * TILE_Y3) - 1
+                yp = (y0 - R < yl.size) or (y1 + R >= ny - yr.size)
+                for tz in range(ntz):
+                    z0, z1 = tz * TILE_Z, min(nz, (tz + 1) * TILE_Z) - 1
+                    zp = (z0 - R < zl.size) or (z1 + R >= nz - zr.size)
+                    flags[(tx * nty + ty) * ntz + tz] = 1 if (xp or yp or zp) else 0
+        return (ntx, nty, ntz), flags
+
+    def _tile_of_pos(self, s_tiles, pos):
+        ntx, nty, ntz = s_tiles
+        if self.dim == 2:
+            return (pos[:, 0] // self.tile_x) * ntz + pos[:, 1] // TILE_Z
+        return ((pos[:, 0] // self.tile_x) * nty + pos[:, 1] // TILE_Y3) * ntz + pos[:, 2] // TILE_Z
+
+    def _build_tables(self):
+        dev = self.dev
+        T = self.dtype
+        self.offs = []
+        off = 0
+        tiles_info = []
+        for s in self.shots:
+            self.offs.append(off)
+            off += s.npts
+            tiles_info.append(self._tiles_of(s))
+        self.total_n = off
+        self.tiles_info = tiles_info
+
+        # tile table: interleave the shots of one tile when they are alike,
+        # else shot-major (active prefix by kshift)
+        same = all(s.shape == self.shots[0].shape and s.window == self.shots[0].window
+                   for s in self.shots) and len(set(self.kshift)) == 1
+        rows = []
+        if same:
+            ntile = len(tiles_info[0][1])
+            for t in range(ntile):
+                for si in range(self.n_shots):
+                    rows.append((si, t, int(tiles_info[si][1][t]), 0))
+        else:
+            for si in range(self.n_shots):
+                fl = tiles_info[si][1]
+                for t in range(len(fl)):
+                    rows.append((si, t, int(fl[t]), 0))
+        tiles = np.asarray(rows, dtype=np.int32).reshape(-1, 4)
+        self.n_tiles = tiles.shape[0]
+        tile_index = {(int(r[0]), int(r[1])): i for i, r in enumerate(tiles)}
+        self._tile_index = tile_index
+        self.tiles = torch.from_numpy(tiles.reshape(-1).copy()).to(dev)
+
+        # active prefix per slot
+        shot_tiles = np.array([len(ti[1]) for ti in tiles_info])
+        shot_recs = np.array([s.nrec for s in self.shots])
+        ks = np.array(self.kshift)
+        slots = np.arange(self.max_nt)
+        active = ks[None, :] <= slots[:, None]
+        if same:
+            self.active_tiles = np.full(self.max_nt, self.n_tiles, dtype=np.int32)
+        else:
+            self.active_tiles = (active * shot_tiles[None, :]).sum(1).astype(np.int32)
+        self.active_recs = (active * shot_recs[None, :]).sum(1).astype(np.int32)
+
+        # PML arrays and model windows (shared tensors where equal)
+        self._keep = []
+        self.fwd_inj = self._injection_tables([s.src for s in self.shots])
+        has_adj = all(s.adj is not None for s in self.shots)
+        self.adj_inj = self._injection_tables([s.adj for s in self.shots]) if has_adj else None
+
+        # receivers
+        recs, ridx, rw = [], [], []
+        e = 0
+        for si, s in enumerate(self.shots):
+            idx, chan, w = s.rec
+            order = np.argsort(chan, kind="stable")
+            idx, chan, w = idx[order], chan[order], w[order]
+            counts = np.bincount(chan, minlength=s.nrec)
+            starts = np.concatenate([[0], np.cumsum(counts)])
+            for r in range(s.nrec):
+                recs.append((si, r, e + starts[r], e + starts[r + 1]))
+            ridx.append(idx)
+            rw.append(w)
+            e += idx.size
+        self.n_rec = len(recs)
+        self.recs = torch.from_numpy(np.asarray(recs, dtype=np.int32).reshape(-1)).to(dev)
+        self.rec_idx = torch.from_numpy(np.concatenate(ridx).astype(np.int64) if ridx else np.zeros(0, np.int64)).to(dev)
+        self.rec_w = torch.from_numpy(np.concatenate(rw) if rw else np.zeros(0)).to(dev)
+
+        # per-shot device arrays
+        self.pml_dev = []
+        for s in self.shots:
+            arrs = []
+            for l, r in s.kpml:
+                arrs.append(torch.from_numpy(l.astype(T)).to(dev) if l.size else None)
+                arrs.append(torch.from_numpy(r.astype(T)).to(dev) if r.size else None)
+            self.pml_dev.append(arrs)
+        self.amp_dev = [torch.from_numpy(s.amp).to(dev) for s in self.shots]
+        self.traces = [torch.zeros((s.nt, s.nrec), dtype=torch.float64, device=dev) for s in self.shots]
+        self.shots_fwd = self._shot_table(self.amp_dev, [s.amp.shape[1] for s in self.shots])
+        self.residual_dev = None
+        self.shots_adj = None
+
+    def _injection_tables(self, entries):
+        ptr_counts = np.zeros(self.n_tiles, dtype=np.int64)
+        pos_all, w_all, tile_all = [], [], []
+        for si, (s, e) in enumerate(zip(self.shots, entries)):
+            pos, chan, w = e
+            if pos.shape[0] == 0:
+                continue
+            if np.any(pos < 0) or np.any(pos >= np.asarray(s.shape)[None, :]):
+                raise ValueError("injection entry outside the padded grid")
+            tloc = self._tile_of_pos(self.tiles_info[si][0], pos)
+            tglob = np.array([self._tile_index[(si, int(t))] for t in tloc], dtype=np.int64)
+            if self.dim == 2:
+                p4 = np.stack([pos[:, 0], np.zeros_like(pos[:, 0]), pos[:, 1], chan], axis=1)
+            else:
+                p4 = np.stack([pos[:, 0], pos[:, 1], pos[:, 2], chan], axis=1)
+            # within a tile: by x, keeping the reference's accumulation order
+            key = np.lexsort((np.arange(pos.shape[0]), pos[:, 0], tglob))
+            pos_all.append(p4[key])
+            w_all.append(w[key])
+            tile_all.append(tglob[key])
+        if pos_all:
+            tile_all = np.concatenate(tile_all)
+            key = np.argsort(tile_all, kind="stable")
+            pos_all = np.concatenate(pos_all)[key].astype(np.int32)
+            w_all = np.concatenate(w_all)[key]
+            tile_all = tile_all[key]
+            ptr_counts = np.bincount(tile_all, minlength=self.n_tiles)
+        else:
+            pos_all = np.zeros((0, 4), np.int32)
+            w_all = np.zeros(0)
+        ptr = np.concatenate([[0], np.cumsum(ptr_counts)]).astype(np.int32)
+        return (torch.from_numpy(ptr).to(self.dev),
+                torch.from_numpy(np.ascontiguousarray(pos_all).reshape(-1)).to(self.dev),
+                torch.from_numpy(np.ascontiguousarray(w_all, dtype=np.float64)).to(self.dev))
+
+    def _shot_table(self, amps, namps):
+        arr = (N.Shot * self.n_shots)()
+        for si, s in enumerate(self.shots):
+            d = arr[si]
+            d.nx = s.shape[0]
+            d.ny = s.shape[1] if self.dim == 3 else 1
+            d.nz = s.shape[-1]
+            d.nt = s.nt
+            d.kshift = self.kshift[si]
+            ntx, nty, ntz = self.tiles_info[si][0]
+            d.ntx, d.nty, d.ntz = ntx, nty, ntz
+            d.off = self.offs[si]
+            d.model = self.model.data_ptr()
+            d.mny = self.model_shape[1] if self.dim == 3 else 1
+            d.mnz = self.model_shape[-1]
+            wx = s.window[0]
+            wz = s.window[-1]
+            wy = s.window[1] if self.dim == 3 else (0, 0, 0)
+            d.mx_off, d.mx_lo, d.mx_hi = wx
+            d.my_off, d.my_lo, d.my_hi = wy
+            d.mz_off, d.mz_lo, d.mz_hi = wz
+            pm = self.pml_dev[si]
+            if self.dim == 2:
+                xl, xr, zl, zr = pm
+                yl = yr = None
+                (nxl, nxr), (nzl, nzr) = [(l.size, r.size) for l, r in s.kpml]
+                nyl = nyr = 0
+            else:
+                xl, xr, yl, yr, zl, zr = pm
+                (nxl, nxr), (nyl, nyr), (nzl, nzr) = [(l.size, r.size) for l, r in s.kpml]
+            for name, t in (("xl", xl), ("xr", xr), ("yl", yl), ("yr", yr), ("zl", zl), ("zr", zr)):
+                setattr(d, name, t.data_ptr() if t is not None else None)
+            d.nxl, d.nxr, d.nyl, d.nyr, d.nzl, d.nzr = nxl, nxr, nyl, nyr, nzl, nzr
+            d.amp = amps[si].data_ptr() if amps[si] is not None else None
+            d.namp = namps[si]
+            d.nrec = s.nrec
+            d.trace = self.traces[si].data_ptr()
+        raw = np.frombuffer(bytes(arr), dtype=np.uint8).copy()
+        return torch.from_numpy(raw).to(self.dev)
+
+    # ------------------------------------------------------------------- state
+    def _fields(self):
+        return ("u", "phix", "phiz") if self.dim == 2 else ("u", "phix", "phiy", "phiz", "psi")
+
+    def _new_state(self):
+        return {f: torch.zeros((2, self.total_n), dtype=self.tdtype, device=self.dev) for f in self._fields()}
+
+    def _alloc_state(self):
+        self.fstate = self._new_state()
+        self.qstate = None
+        self.dw = None
+        self.dw_slots = 0
+        self.ic = None
+
+    def state_bytes(self):
+        return len(self._fields()) * 2 * self.total_n * self.dtype.itemsize
+
+    def _batch(self, state, adjoint, ic=None, dw=None):
+        b = N.Batch()
+        b.dtype = 1 if self.dtype == np.float64 else 0
+        b.dim = self.dim
+        b.order = self.order
+        b.tile_x = self.tile_x
+        b.tile_y = TILE_Y3
+        b.dt = self.dt
+        b.dx = self.deltas[0]
+        b.dy = self.deltas[1] if self.dim == 3 else 1.0
+        b.dz = self.deltas[-1]
+        b.n_shots = self.n_shots
+        b.shots = (self.shots_adj if adjoint else self.shots_fwd).data_ptr()
+        b.n_tiles = self.n_tiles
+        b.tiles = self.tiles.data_ptr()
+        inj = self.adj_inj if adjoint else self.fwd_inj
+        b.inj_ptr, b.inj_pos, b.inj_w = inj[0].data_ptr(), inj[1].data_ptr(), inj[2].data_ptr()
+        b.n_rec = 0 if adjoint else self.n_rec
+        b.recs = self.recs.data_ptr() if self.n_rec else None
+        b.rec_idx = self.rec_idx.data_ptr() if self.n_rec else None
+        b.rec_w = self.rec_w.data_ptr() if self.n_rec else None
+        for f in ("u", "phix", "phiy", "phiz", "psi"):
+            arr = getattr(b, f)
+            if f in state:
+                arr[0] = state[f][0].data_ptr()
+                arr[1] = state[f][1].data_ptr()
+        b.total_n = self.total_n
+        b.dw = dw.data_ptr() if dw is not None else None
+        b.ic = ic.data_ptr() if ic is not None else None
+        b.imaging_period = self.imaging_period
+        return b
+
+    def _run(self, b, mode, j0, j1, direction, seg_base, record=True):
+        at = np.ascontiguousarray(self.active_tiles)
+        ar = np.ascontiguousarray(self.active_recs if record else np.zeros_like(self.active_recs))
+        N.call("pysit_amd_batch_run", ctypes.byref(b), mode, j0, j1, direction, seg_base,
+               at.ctypes.data_as(ctypes.c_void_p), ar.ctypes.data_as(ctypes.c_void_p), _stream())
+
+    # ---------------------------------------------------------- checkpointing
+    def plan_segments(self, keep_dw):
+        """Slots per dWaveOp segment K (K = max_nt means every step is stored).
+
+        The store needs K * total_n values; each checkpoint keeps the whole
+        state.  K is the largest value whose store plus checkpoints fit the
+        memory budget (default: 80% of the free device memory)."""
+        if not keep_dw:
+            return self.max_nt
+        budget = self.memory_budget
+        if budget is None:
+            free, _ = torch.cuda.mem_get_info(self.dev)
+            budget = int(0.8 * free)
+        item = self.dtype.itemsize
+        per_slot = self.total_n * item
+        ckpt = self.state_bytes()
+        if self.max_nt * per_slot <= budget:
+            return self.max_nt
+        # nseg segments: nseg checkpoints, one working state, a K-slot store
+        best = None
+        for nseg in range(2, self.max_nt + 1):
+            K = -(-self.max_nt // nseg)
+            if K * per_slot + (nseg + 1) * ckpt <= budget:
+                best = K
+                break
+            if (nseg + 1) * ckpt > budget:
+                break
+        if best is None:
+            raise MemoryError("memory budget too small for the checkpoint schedule")
+        return best
+
+    def _ensure_dw(self, K):
+        if self.dw is None or self.dw_slots < K:
+            self.dw = None
+            torch.cuda.empty_cache()
+            self.dw = torch.empty((K, self.total_n), dtype=self.tdtype, device=self.dev)
+            self.dw_slots = K
+
+    # ----------------------------------------------------------------- forward
+    def forward(self, keep_dw=False, segment=None):
+        """Run every shot forward from rest, recording traces.
+
+        With ``keep_dw`` the state is prepared for :meth:`migrate`: either the
+        whole dWaveOp history is stored (one segment) or checkpoints are kept
+        at segment boundaries.  Returns the per-shot traces (device FP64, in
+        the caller's shot order)."""
+        for t in self.traces:
+            t.zero_()
+        for f in self.fstate.values():
+            f.zero_()
+        K = segment if segment else self.plan_segments(keep_dw)
+        self.K = K
+        self.ckpts = []
+        if keep_dw and K >= self.max_nt:
+            self._ensure_dw(self.max_nt)
+            b = self._batch(self.fstate, False, dw=self.dw)
+            self._run(b, 1, 0, self.max_nt, 1, 0)
+        elif keep_dw:
+            b = self._batch(self.fstate, False)
+            for j0 in range(0, self.max_nt, K):
+                self.ckpts.append({f: v.clone() for f, v in self.fstate.items()})
+                self._run(b, 0, j0, min(self.max_nt, j0 + K), 1, j0)
+        else:
+            b = self._batch(self.fstate, False)
+            self._run(b, 0, 0, self.max_nt, 1, 0)
+        self.have_dw = keep_dw
+        return self.traces_in_caller_order()
+
+    def traces_in_caller_order(self):
+        out = [None] * self.n_shots
+        for pos, orig in enumerate(self.order_idx):
+            out[orig] = self.traces[pos]
+        return out
+
+    # ----------------------------------------------------------------- migrate
+    def migrate(self, residuals):
+        """Imaging condition -dt*p*sum_k q_k dWaveOp_k of every shot (padded,
+        FP64, concatenated at ``offs``).  ``residuals`` are per-shot (nt, nrec)
+        operands in the caller's shot order (numpy or device tensors)."""
+        if not getattr(self, "have_dw", False):
+            raise RuntimeError("migrate needs a forward(keep_dw=True) first")
+        if self.adj_inj is None:
+            raise RuntimeError("the shots carry no adjoint injection entries")
+        res = [None] * self.n_shots
+        for pos, orig in enumerate(self.order_idx):
+            r = residuals[orig]
+            s = self.shots[pos]
+            r = torch.as_tensor(r, dtype=torch.float64, device=self.dev).contiguous()
+            if tuple(r.shape) != (s.nt, s.nrec):
+                raise ValueError(f"residual of shot {orig} has shape {tuple(r.shape)}, expected {(s.nt, s.nrec)}")
+            res[pos] = r
+        self.residual_dev = res
+        self.shots_adj = self._shot_table(res, [s.nrec for s in self.shots])
+        if self.qstate is None:
+            self.qstate = self._new_state()
+        for f in self.qstate.values():
+            f.zero_()
+        self.ic = torch.zeros(self.total_n, dtype=torch.float64, device=self.dev)
+        K = self.K
+        qb = self._batch(self.qstate, True, ic=self.ic, dw=None)
+        if not self.ckpts:
+            qb.dw = self.dw.data_ptr()
+            self._run(qb, 2, 0, self.max_nt, -1, 0, record=False)
+        else:
+            self._ensure_dw(K)
+            qb.dw = self.dw.data_ptr()
+            work = self._new_state()
+            fb = self._batch(work, False, dw=self.dw)
+            for si in range(len(self.ckpts) - 1, -1, -1):
+                j0 = si * K
+                j1 = min(self.max_nt, j0 + K)
+                for f, v in self.ckpts[si].items():
+                    work[f].copy_(v)
+                self._run(fb, 1, j0, j1, 1, j0, record=False)
+                self._run(qb, 2, j0, j1, -1, j0, record=False)
+        return self.ic
+
+    def image(self, si_caller, scale=1.0):
+        """Padded FP64 image of one shot (caller order), a view of the accumulator."""
+        pos = self.order_idx.index(si_caller)
+        s = self.shots[pos]
+        return self.ic[self.offs[pos]:self.offs[pos] + s.npts].view(s.shape)
+
+    def fold(self, si_caller, pads, out, out_origin=None, scale=1.0, accumulate=True):
+        """Fold one shot's image onto ``out`` (device FP64 tensor holding the
+        full unpadded model grid); ``out_origin`` is the shot window's first
+        node in ``out`` (default 0)."""
+        pos = self.order_idx.index(si_caller)
+        s = self.shots[pos]
+        img = self.ic[self.offs[pos]:]
+        origin = out_origin or (0,) * self.dim
+        if self.dim == 2:
+            (lx, rx), (lz, rz) = pads
+            ly = ry = 0
+            onz = out.shape[-1]
+            ony = 1
+            o = out.reshape(-1)[origin[0] * onz + origin[1]:]
+            nx, nz = s.shape
+            ny = 1
+        else:
+            (lx, rx), (ly, ry), (lz, rz) = pads
+            ony, onz = out.shape[1], out.shape[2]
+            o = out.reshape(-1)[(origin[0] * ony + origin[1]) * onz + origin[2]:]
+            nx, ny, nz = s.shape
+        N.call("pysit_amd_fold", self.dim, nx, ny, nz, lx, rx, ly, ry, lz, rz, _ptr(img), scale,
+               _ptr(o), ony, onz, 1 if accumulate else 0, _stream())

@@ -1,0 +1,138 @@
+"""Synthetic propagation problems shared by the tests.
+
+Each problem is built twice from the same numbers: once as the oracle's
+``Problem`` (flat entry lists) and once as the device ``ShotGeometry``.
+The PML profiles follow the reference's closed form (pysit/core/domain.py:
+236-254 with mesh.py:749-782): left sigma[j] = A((n-j)/n)^2, right
+sigma[j] = A((j+1)/n)^2.
+"""
+import numpy as np
+
+
+def pml_pair(n, amp):
+    if n == 0:
+        return np.zeros(0), np.zeros(0)
+    s = amp * np.linspace(0.0, 1.0, n + 1) ** 2
+    return s[::-1][:-1].copy(), s[1:].copy()
+
+
+def mesh_sigma(n_axis, left, right):
+    sig = np.zeros(n_axis)
+    if left.size:
+        sig[:left.size] += left
+    if right.size:
+        sig[n_axis - right.size:] += right
+    return sig
+
+
+def gaussian_patch(center, shape, width, rng, dim):
+    """Entries of a (2w+1)^dim patch around center clipped to the grid."""
+    rngs = [np.arange(max(0, c - width), min(n, c + width + 1)) for c, n in zip(center, shape)]
+    grid = np.meshgrid(*rngs, indexing="ij")
+    pos = np.stack([g.reshape(-1) for g in grid], axis=1)
+    d2 = sum((pos[:, i] - center[i]) ** 2 for i in range(dim))
+    w = np.exp(-0.5 * d2 / max(1.0, width / 2) ** 2) * (1.0 + 0.1 * rng.random(pos.shape[0]))
+    return pos, w
+
+
+class Synthetic:
+    """One shot on a padded grid with PML of widths `pml` per axis side."""
+
+    def __init__(self, shape, pml, deltas, nt, order, seed=0, amp_pml=60.0, n_src=1, n_rec=7,
+                 window=None, model=None, dt=None):
+        rng = np.random.default_rng(seed)
+        self.shape = tuple(shape)
+        dim = len(shape)
+        self.dim = dim
+        self.deltas = tuple(deltas)
+        self.order = order
+        self.nt = nt
+        self.pml = pml  # per axis (nl, nr)
+        self.kpml = [pml_pair(nl, amp_pml) if nl else (np.zeros(0), np.zeros(0)) for nl, nr in pml]
+        # allow different right width
+        self.kpml = []
+        for (nl, nr) in pml:
+            l = pml_pair(nl, amp_pml)[0] if nl else np.zeros(0)
+            r = pml_pair(nr, amp_pml * 1.3)[1] if nr else np.zeros(0)
+            self.kpml.append((l, r))
+        self.msig = [mesh_sigma(n, l, r) for n, (l, r) in zip(shape, self.kpml)]
+        # model: unpadded interior sizes, edge padded by clamping
+        inner = tuple(n - nl - nr for n, (nl, nr) in zip(shape, pml))
+        if model is None:
+            base = 1.0 + 0.3 * rng.random(inner)
+            model = base
+        self.model = np.asarray(model, dtype=np.float64)
+        if window is None:
+            window = [(nl, 0, n_in - 1) for (nl, nr), n_in in zip(pml, inner)]
+        self.window = window
+        self.Cpad = self._pad_model()
+        vmax = self.Cpad.max()
+        self.dt = dt if dt is not None else (1.0 / 6.0) * min(deltas) / vmax
+        # sources
+        pos_all, chan_all, w_all = [], [], []
+        for c in range(n_src):
+            center = [int(rng.integers(pml[i][0] + 2, shape[i] - pml[i][1] - 2)) for i in range(dim)]
+            pos, w = gaussian_patch(center, shape, 2, rng, dim)
+            pos_all.append(pos)
+            chan_all.append(np.full(pos.shape[0], c))
+            w_all.append(w)
+        self.src = (np.concatenate(pos_all), np.concatenate(chan_all), np.concatenate(w_all))
+        t = np.arange(nt + 1) * self.dt
+        f0 = 1.0 / (12 * self.dt)
+        amp = np.zeros((nt + 1, n_src))
+        for c in range(n_src):
+            tau = t - (1.5 + 0.3 * c) / f0
+            x = (np.pi * f0 * tau) ** 2
+            amp[:, c] = (1 - 2 * x) * np.exp(-x)
+        self.amp = amp
+        # receivers: a line along x (2D) or a patch (3D) near the top
+        rec_pos, rec_chan, rec_w = [], [], []
+        zr = pml[-1][0] + 2
+        for r in range(n_rec):
+            if dim == 2:
+                xr = int(np.linspace(pml[0][0] + 1, shape[0] - pml[0][1] - 2, n_rec)[r])
+                center = [xr, zr]
+            else:
+                xr = int(np.linspace(pml[0][0] + 1, shape[0] - pml[0][1] - 2, n_rec)[r])
+                yr = int(rng.integers(pml[1][0] + 1, shape[1] - pml[1][1] - 1))
+                center = [xr, yr, zr]
+            pos, w = gaussian_patch(center, shape, 1, rng, dim)
+            rec_pos.append(pos)
+            rec_chan.append(np.full(pos.shape[0], r))
+            rec_w.append(w)
+        rp = np.concatenate(rec_pos)
+        self.rec_pos = rp
+        self.rec_chan = np.concatenate(rec_chan)
+        self.rec_w = np.concatenate(rec_w)
+        self.n_rec = n_rec
+        self.rec_idx = np.ravel_multi_index(tuple(rp.T), shape)
+
+    def _pad_model(self):
+        idx = []
+        for i, (off, lo, hi) in enumerate(self.window):
+            idx.append(np.clip(np.arange(self.shape[i]) - off, lo, hi))
+        return self.model[np.ix_(*idx)]
+
+    def flat(self, pos):
+        return np.ravel_multi_index(tuple(np.asarray(pos).T), self.shape)
+
+    def oracle_problem(self):
+        from oracle.oracle import Problem
+        src = (self.flat(self.src[0]), self.src[1], self.src[2])
+        rec = (self.rec_idx, self.rec_chan, self.rec_w * 0.5)
+        adj = (self.rec_idx, self.rec_chan, self.rec_w)
+        return Problem(self.shape, self.deltas, self.dt, self.nt, self.Cpad, self.kpml, self.msig,
+                       src, self.amp, rec, self.n_rec, adj=adj, order=self.order, pad=self.pml)
+
+    def geometry(self):
+        from pysit_amd.device import ShotGeometry
+        return ShotGeometry(self.shape, self.kpml, self.window, self.nt, self.src, self.amp,
+                            (self.rec_idx, self.rec_chan, self.rec_w * 0.5), self.n_rec,
+                            adj=(self.rec_pos, self.rec_chan, self.rec_w))
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))

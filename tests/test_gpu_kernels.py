@@ -1,0 +1,209 @@
+"""HIP kernels against the CPU oracle (GPU tests, called through the C ABI)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.problems import Synthetic, pml_pair, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev_arrays(gpu, arrays, dtype):
+    import torch
+    t = [torch.from_numpy(np.ascontiguousarray(a, dtype=dtype)).to(gpu) for a in arrays]
+    return t
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() else ctypes.c_void_p(0)
+
+
+def _random_state_2d(rng, nx, nz):
+    f = [rng.standard_normal(nx * nz) for _ in range(6)]
+    f[4] = 1.0 + 0.5 * rng.random(nx * nz)  # C
+    return f
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("order", [2, 4, 6, 8])
+@pytest.mark.parametrize("shape", [(23, 19), (70, 133)])
+def test_step2d_matches_oracle(gpu, dtype, order, shape):
+    import torch
+    from pysit_amd import _native as N
+    rng = np.random.default_rng(order + shape[0])
+    nx, nz = shape
+    f = _random_state_2d(rng, nx, nz)
+    xl, _ = pml_pair(5, 40.0)
+    _, xr = pml_pair(6, 55.0)
+    zl, _ = pml_pair(4, 70.0)
+    _, zr = pml_pair(3, 30.0)
+    dt, dx, dz = 1e-3, 0.011, 0.013
+    ref = O.step2d(order, *f, xl, xr, zl, zr, dt, dx, dz, nx, nz, dtype=dtype)
+    d = _dev_arrays(gpu, f + [xl, xr, zl, zr], dtype)
+    outs = [torch.zeros(nx * nz, dtype=d[0].dtype, device=gpu) for _ in range(3)]
+    name = "pysit_amd_cda_step2d_" + ("f64" if dtype == np.float64 else "f32")
+    args = [order] + [_p(t) for t in d[:6]]
+    for t in d[6:]:
+        args += [_p(t), t.numel()]
+    args += [dt, dx, dz, nx, nz] + [_p(t) for t in outs] + [None]
+    N.call(name, *args)
+    torch.cuda.synchronize()
+    tol = 1e-12 if dtype == np.float64 else 2e-6
+    for o, r in zip(outs, ref):
+        got = o.cpu().numpy().astype(np.float64)
+        scale = max(1.0, np.abs(r).max())
+        assert np.abs(got - r).max() <= tol * scale * 10, (name, order, np.abs(got - r).max())
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("order", [2, 8])
+def test_step3d_matches_oracle(gpu, dtype, order):
+    import torch
+    from pysit_amd import _native as N
+    rng = np.random.default_rng(7 + order)
+    nx, ny, nz = 13, 17, 71
+    n = nx * ny * nz
+    f = [rng.standard_normal(n) for _ in range(8)]
+    f[6] = 1.0 + 0.5 * rng.random(n)
+    pmls = [pml_pair(3, 40.0)[0], pml_pair(4, 50.0)[1], pml_pair(2, 60.0)[0], pml_pair(3, 45.0)[1],
+            pml_pair(5, 35.0)[0], pml_pair(2, 65.0)[1]]
+    dt, dx, dy, dz = 1e-3, 0.011, 0.012, 0.013
+    ref = O.step3d(order, *f, *pmls, dt, dx, dy, dz, nx, ny, nz, dtype=dtype)
+    d = _dev_arrays(gpu, f + pmls, dtype)
+    outs = [torch.zeros(n, dtype=d[0].dtype, device=gpu) for _ in range(5)]
+    name = "pysit_amd_cda_step3d_" + ("f64" if dtype == np.float64 else "f32")
+    args = [order] + [_p(t) for t in d[:8]]
+    for t in d[8:]:
+        args += [_p(t), t.numel()]
+    args += [dt, dx, dy, dz, nx, ny, nz] + [_p(t) for t in outs] + [None]
+    N.call(name, *args)
+    torch.cuda.synchronize()
+    tol = 1e-12 if dtype == np.float64 else 2e-6
+    for o, r in zip(outs, ref):
+        got = o.cpu().numpy().astype(np.float64)
+        scale = max(1.0, np.abs(r).max())
+        assert np.abs(got - r).max() <= tol * scale * 10, (name, order, np.abs(got - r).max())
+
+
+def _batch(probs, dtype, **kw):
+    from pysit_amd.device import DeviceBatch
+    p0 = probs[0]
+    return DeviceBatch([p.geometry() for p in probs], p0.model, p0.deltas, p0.dt, p0.order,
+                       dtype=dtype, **kw)
+
+
+@pytest.mark.parametrize("order", [4, 8])
+def test_batch_forward_and_migrate_fp64_2d(gpu, order):
+    probs = [Synthetic((60, 75), ((6, 7), (5, 8)), (0.01, 0.012), 90, order, seed=s, n_src=1 + s)
+             for s in range(3)]
+    for p in probs[1:]:
+        p.model = probs[0].model
+        p.Cpad = probs[0].Cpad
+        p.dt = probs[0].dt
+    b = _batch(probs, np.float64)
+    traces = b.forward(keep_dw=True)
+    rng = np.random.default_rng(5)
+    res = [rng.standard_normal((p.nt, p.n_rec)) for p in probs]
+    ic = b.migrate(res).cpu().numpy()
+    for i, p in enumerate(probs):
+        op = p.oracle_problem()
+        # sources differ per shot: rebuild the oracle amplitude set for that shot
+        ref = op.forward(simdata=True)["simdata"]
+        assert rel_l2(traces[i].cpu().numpy(), ref) < 1e-11
+        img_ref = op.migrate(res[i], fold=False)
+        img = b.image(i).cpu().numpy().reshape(-1) * (-p.dt)
+        assert rel_l2(img, img_ref) < 1e-10
+
+
+def test_batch_checkpointed_equals_store_all(gpu):
+    p = Synthetic((50, 70), ((5, 5), (4, 6)), (0.01, 0.01), 61, 8, seed=11)
+    rng = np.random.default_rng(1)
+    res = [rng.standard_normal((p.nt, p.n_rec))]
+    b1 = _batch([p], np.float64)
+    b1.forward(keep_dw=True)
+    ic1 = b1.migrate(res).cpu().numpy()
+    b2 = _batch([p], np.float64)
+    b2.forward(keep_dw=True, segment=7)
+    assert len(b2.ckpts) == 9
+    ic2 = b2.migrate(res).cpu().numpy()
+    assert np.array_equal(ic1, ic2)
+
+
+def test_batch_uneven_shots_2d(gpu):
+    base = Synthetic((80, 60), ((6, 6), (5, 5)), (0.01, 0.01), 100, 8, seed=3)
+    probs = []
+    for s, (w0, w1, nt) in enumerate([(0, 50, 100), (20, 67, 73), (5, 40, 88)]):
+        ncols = w1 - w0
+        window = [(6 - w0, w0, w1 - 1), (5, 0, base.model.shape[1] - 1)]
+        p = Synthetic((ncols + 12, 60), ((6, 6), (5, 5)), (0.01, 0.01), nt, 8, seed=20 + s,
+                      window=window, model=base.model, dt=base.dt)
+        probs.append(p)
+    from pysit_amd.device import DeviceBatch
+    b = DeviceBatch([p.geometry() for p in probs], base.model, base.deltas, base.dt, 8, dtype=np.float64)
+    traces = b.forward(keep_dw=True)
+    rng = np.random.default_rng(9)
+    res = [rng.standard_normal((p.nt, p.n_rec)) for p in probs]
+    b.migrate(res)
+    for i, p in enumerate(probs):
+        op = p.oracle_problem()
+        ref = op.forward()["simdata"]
+        assert rel_l2(traces[i].cpu().numpy(), ref) < 1e-11
+        img_ref = op.migrate(res[i], fold=False)
+        img = b.image(i).cpu().numpy().reshape(-1) * (-p.dt)
+        assert rel_l2(img, img_ref) < 1e-10
+
+
+def test_batch_fp32_close_to_fp64_oracle(gpu):
+    p = Synthetic((90, 110), ((10, 10), (10, 10)), (0.01, 0.01), 400, 8, seed=4)
+    b = _batch([p], np.float32)
+    tr = b.forward(keep_dw=True)[0].cpu().numpy()
+    op = p.oracle_problem()
+    ref = op.forward()["simdata"]
+    assert rel_l2(tr, ref) < 1e-4
+    rng = np.random.default_rng(2)
+    res = [ref - 0.9 * ref + 0.01 * rng.standard_normal(ref.shape) * np.abs(ref).max()]
+    b.migrate(res)
+    img = b.image(0).cpu().numpy().reshape(-1) * (-p.dt)
+    img_ref = op.migrate(res[0], fold=False)
+    assert rel_l2(img, img_ref) < 1e-3
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-10), (np.float32, 1e-3)])
+def test_batch_3d(gpu, dtype, tol):
+    p = Synthetic((20, 22, 70), ((4, 4), (3, 5), (4, 3)), (0.01, 0.011, 0.012), 40, 8, seed=5, n_rec=5)
+    b = _batch([p], dtype)
+    tr = b.forward(keep_dw=True)[0].cpu().numpy()
+    op = p.oracle_problem()
+    ref = op.forward()["simdata"]
+    assert rel_l2(tr, ref) < tol
+    rng = np.random.default_rng(3)
+    res = [rng.standard_normal(ref.shape)]
+    b.migrate(res)
+    img = b.image(0).cpu().numpy().reshape(-1) * (-p.dt)
+    img_ref = op.migrate(res[0], fold=False)
+    assert rel_l2(img, img_ref) < tol * 10
+
+
+def test_fold_matches_oracle(gpu):
+    import torch
+    from pysit_amd import _native as N
+    rng = np.random.default_rng(0)
+    for shape, pads in [((30, 25), ((5, 4), (3, 6))), ((12, 14, 16), ((2, 3), (4, 1), (3, 3)))]:
+        a = rng.standard_normal(int(np.prod(shape)))
+        ref = O.fold_padding(a, shape, pads)
+        inner = tuple(n - l - r for n, (l, r) in zip(shape, pads))
+        t = torch.from_numpy(a).to(gpu)
+        out = torch.zeros(inner, dtype=torch.float64, device=gpu)
+        dim = len(shape)
+        if dim == 2:
+            args = (2, shape[0], 1, shape[1], pads[0][0], pads[0][1], 0, 0, pads[1][0], pads[1][1])
+            ony, onz = 1, inner[1]
+        else:
+            args = (3, *shape, pads[0][0], pads[0][1], pads[1][0], pads[1][1], pads[2][0], pads[2][1])
+            ony, onz = inner[1], inner[2]
+        N.call("pysit_amd_fold", *args, ctypes.c_void_p(t.data_ptr()), 1.0,
+               ctypes.c_void_p(out.data_ptr()), ony, onz, 0, None)
+        torch.cuda.synchronize()
+        assert np.allclose(out.cpu().numpy().reshape(-1), ref, rtol=1e-13, atol=1e-13)
