@@ -20,6 +20,12 @@
 // the step is bound by HBM bandwidth.
 #include "common.h"
 
+// No implicit FP contraction: every fused multiply-add below is written out,
+// so all kernel modes (propagate, store dWaveOp, adjoint) evaluate the step
+// with the same roundings and a checkpoint recompute reproduces the stored
+// history bit for bit.
+#pragma clang fp contract(off)
+
 namespace pa {
 
 template <typename T>
@@ -133,11 +139,11 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
         T lx = T(0), lz = T(0);
 #pragma unroll
         for (int j = R; j >= 1; --j) {
-            lx += P.cx.d2[j] * (q[R + j] + q[R - j]);
-            lz += P.cz.d2[j] * (su[R + lane + j] + su[R + lane - j]);
+            lx = fma(P.cx.d2[j], q[R + j] + q[R - j], lx);
+            lz = fma(P.cz.d2[j], su[R + lane + j] + su[R + lane - j], lz);
         }
-        lx += P.cx.d2[0] * uc;
-        lz += P.cz.d2[0] * uc;
+        lx = fma(P.cx.d2[0], uc, lx);
+        lz = fma(P.cz.d2[0], uc, lz);
         const T lap = lx + lz;
         const T c2 = cv * cv;
         const bool edge = (x == 0) | (x == nx - 1) | (z == 0) | (z >= nz - 1);
@@ -145,16 +151,16 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
         T up, pxn = T(0), pzn = T(0), dw;
         if (!full) {
             const T src = f + lap;
-            up = P.tc.dt2 * c2 * src - (um - T(2) * uc);
+            up = fma(P.tc.dt2, c2 * src, T(2) * uc - um);
             dw = c2 * src;
         } else {
             T dux = T(0), duz = T(0), dpx = T(0), dpz = T(0);
 #pragma unroll
             for (int j = R; j >= 1; --j) {
-                dux += P.cx.d1[j - 1] * (q[R + j] - q[R - j]);
-                duz += P.cz.d1[j - 1] * (su[R + lane + j] - su[R + lane - j]);
-                dpx += P.cx.d1[j - 1] * (pq[R + j] - pq[R - j]);
-                dpz += P.cz.d1[j - 1] * (sp[R + lane + j] - sp[R + lane - j]);
+                dux = fma(P.cx.d1[j - 1], q[R + j] - q[R - j], dux);
+                duz = fma(P.cz.d1[j - 1], su[R + lane + j] - su[R + lane - j], duz);
+                dpx = fma(P.cx.d1[j - 1], pq[R + j] - pq[R - j], dpx);
+                dpz = fma(P.cz.d1[j - 1], sp[R + lane + j] - sp[R + lane - j], dpz);
             }
             const T src = ((f + lap) + dpx) + dpz;
             const T sx = sigma_kernel<T>(xl, S.nxl, xr, S.nxr, min(x, nx - 2), nx);
@@ -172,7 +178,7 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
             } else {
                 pxn = pq[R];
                 pzn = pzc;
-                up = P.tc.dt2 * c2 * src - (um - T(2) * uc);
+                up = fma(P.tc.dt2, c2 * src, T(2) * uc - um);
             }
             // dWaveOp with the mesh profile: the step identity gives
             // (u+ - 2u + u-)/dt^2 = c^2 src - sum(sk)(u+ - u-)/(2dt) - prod(sk) u

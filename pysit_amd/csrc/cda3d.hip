@@ -17,6 +17,9 @@
 // so that it returns all five fields exactly as the reference defines them.
 #include "common.h"
 
+// No implicit FP contraction (see cda2d.hip): all modes round identically.
+#pragma clang fp contract(off)
+
 namespace pa {
 
 constexpr int TY3 = 8;  // y rows per workgroup (one wavefront each)
@@ -187,13 +190,13 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
             T lx = T(0), lyy = T(0), lz = T(0);
 #pragma unroll
             for (int j = R; j >= 1; --j) {
-                lx += P.cx.d2[j] * (q[R + j] + q[R - j]);
-                lyy += P.cy.d2[j] * (L.u[R + ty + j][R + lane] + L.u[R + ty - j][R + lane]);
-                lz += P.cz.d2[j] * (L.u[R + ty][R + lane + j] + L.u[R + ty][R + lane - j]);
+                lx = fma(P.cx.d2[j], q[R + j] + q[R - j], lx);
+                lyy = fma(P.cy.d2[j], L.u[R + ty + j][R + lane] + L.u[R + ty - j][R + lane], lyy);
+                lz = fma(P.cz.d2[j], L.u[R + ty][R + lane + j] + L.u[R + ty][R + lane - j], lz);
             }
-            lx += P.cx.d2[0] * uc;
-            lyy += P.cy.d2[0] * uc;
-            lz += P.cz.d2[0] * uc;
+            lx = fma(P.cx.d2[0], uc, lx);
+            lyy = fma(P.cy.d2[0], uc, lyy);
+            lz = fma(P.cz.d2[0], uc, lz);
             const T lap = (lx + lyy) + lz;
             const T c2 = cv * cv;
             const bool edge = (x == 0) | (x == nx - 1) | (y == 0) | (y == ny - 1) | (z == 0) |
@@ -201,7 +204,7 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
             T up, dw, pxn = T(0), pyn = T(0), pzn = T(0), psn = T(0);
             if (!full) {
                 const T src = f + lap;
-                up = P.tc.dt2 * c2 * src - (um - T(2) * uc);
+                up = fma(P.tc.dt2, c2 * src, T(2) * uc - um);
                 dw = c2 * src;
             } else {
                 T dux = T(0), duy = T(0), duz = T(0), dpx = T(0), dpy = T(0), dpz = T(0);
@@ -209,15 +212,15 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
 #pragma unroll
                 for (int j = R; j >= 1; --j) {
                     const T ax = P.cx.d1[j - 1], ay = P.cy.d1[j - 1], az = P.cz.d1[j - 1];
-                    dux += ax * (q[R + j] - q[R - j]);
-                    dpx += ax * (qx[R + j] - qx[R - j]);
-                    dsx += ax * (qs[R + j] - qs[R - j]);
-                    duy += ay * (L.u[R + ty + j][R + lane] - L.u[R + ty - j][R + lane]);
-                    dpy += ay * (L.py[R + ty + j][lane] - L.py[R + ty - j][lane]);
-                    dsy += ay * (L.ps[R + ty + j][R + lane] - L.ps[R + ty - j][R + lane]);
-                    duz += az * (L.u[R + ty][R + lane + j] - L.u[R + ty][R + lane - j]);
-                    dpz += az * (L.pz[ty][R + lane + j] - L.pz[ty][R + lane - j]);
-                    dsz += az * (L.ps[R + ty][R + lane + j] - L.ps[R + ty][R + lane - j]);
+                    dux = fma(ax, q[R + j] - q[R - j], dux);
+                    dpx = fma(ax, qx[R + j] - qx[R - j], dpx);
+                    dsx = fma(ax, qs[R + j] - qs[R - j], dsx);
+                    duy = fma(ay, L.u[R + ty + j][R + lane] - L.u[R + ty - j][R + lane], duy);
+                    dpy = fma(ay, L.py[R + ty + j][lane] - L.py[R + ty - j][lane], dpy);
+                    dsy = fma(ay, L.ps[R + ty + j][R + lane] - L.ps[R + ty - j][R + lane], dsy);
+                    duz = fma(az, L.u[R + ty][R + lane + j] - L.u[R + ty][R + lane - j], duz);
+                    dpz = fma(az, L.pz[ty][R + lane + j] - L.pz[ty][R + lane - j], dpz);
+                    dsz = fma(az, L.ps[R + ty][R + lane + j] - L.ps[R + ty][R + lane - j], dsz);
                 }
                 const T src = (((f + lap) + dpx) + dpy) + dpz;
                 const T sxk = sigma_kernel<T>(xl, S.nxl, xr, S.nxr, min(x, nx - 2), nx);
@@ -241,7 +244,7 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
                     pxn = qx[R];
                     pyn = pyc;
                     pzn = pzc;
-                    up = P.tc.dt2 * c2 * src - (um - T(2) * uc);
+                    up = fma(P.tc.dt2, c2 * src, T(2) * uc - um);
                 }
                 const T s1m = (sxm + sym) + szm;
                 const T s2m = (sxm * sym + sxm * szm) + sym * szm;

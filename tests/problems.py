@@ -76,7 +76,9 @@ class Synthetic:
             pos_all.append(pos)
             chan_all.append(np.full(pos.shape[0], c))
             w_all.append(w)
-        self.src = (np.concatenate(pos_all), np.concatenate(chan_all), np.concatenate(w_all))
+        # the reference's point weights integrate to one: ~1/prod(deltas)
+        self.src = (np.concatenate(pos_all), np.concatenate(chan_all),
+                    np.concatenate(w_all) / float(np.prod(deltas)))
         t = np.arange(nt + 1) * self.dt
         f0 = 1.0 / (12 * self.dt)
         amp = np.zeros((nt + 1, n_src))

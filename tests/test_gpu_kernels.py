@@ -172,7 +172,7 @@ def test_batch_fp32_close_to_fp64_oracle(gpu):
 
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-10), (np.float32, 1e-3)])
 def test_batch_3d(gpu, dtype, tol):
-    p = Synthetic((20, 22, 70), ((4, 4), (3, 5), (4, 3)), (0.01, 0.011, 0.012), 40, 8, seed=5, n_rec=5)
+    p = Synthetic((20, 22, 70), ((4, 4), (3, 5), (4, 3)), (0.01, 0.011, 0.012), 90, 8, seed=5, n_rec=5)
     b = _batch([p], dtype)
     tr = b.forward(keep_dw=True)[0].cpu().numpy()
     op = p.oracle_problem()
