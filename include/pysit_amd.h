@@ -160,6 +160,14 @@ typedef struct {
     /* imaging-condition accumulator (FP64, total_n), adjoint only           */
     double *ic;
     int imaging_period;
+    /* optional field store (dtype of the batch), same layout as dw: when not
+     * NULL every launch writes the field of time level k it reads (u_k
+     * forward, q_k adjoint) -- the 'wavefield' / 'adjointfield' returns     */
+    void *fstore;
+    /* Born source (mode 3): model-grid perturbation m1 (dtype of the batch,
+     * same shape and window as the model); rhs_k = -m1 * dWaveOp0_k with
+     * dWaveOp0 read from dw                                                 */
+    const void *born_m1;
 } pysit_amd_batch;
 
 /* Run slots [slot_begin, slot_end) forward (direction = +1) or
@@ -168,6 +176,9 @@ typedef struct {
  *   mode 1: propagate and store dWaveOp for k % imaging_period == 0
  *   mode 2: adjoint: ic += q_k * dWaveOp_k (k % imaging_period == 0), then
  *           propagate; receiver sampling is skipped
+ *   mode 3: Born (linear_forward_model, temporal_modeling.py:389-546):
+ *           propagate with the extra source -m1 * dWaveOp0_k read from dw
+ *           (imaging_period must be 1), recording traces
  * seg_base: slot index of the dWaveOp store's first slot.
  * active_tiles[j], active_recs[j] (HOST arrays indexed by slot, may be NULL =
  * all): how many leading tiles / receivers are active at slot j.

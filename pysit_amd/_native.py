@@ -45,6 +45,7 @@ class Batch(ctypes.Structure):
         ("total_n", _ll),
         ("dw", _vp), ("ic", _vp),
         ("imaging_period", _i),
+        ("fstore", _vp), ("born_m1", _vp),
     ]
 
 

@@ -175,7 +175,9 @@ int pysit_amd_batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int 
     PA_CHECK(B.dtype == 0 || B.dtype == 1, "dtype must be 0 (f32) or 1 (f64)");
     PA_CHECK(B.order == 2 || B.order == 4 || B.order == 6 || B.order == 8,
              "order must be 2, 4, 6 or 8");
-    PA_CHECK(mode >= 0 && mode <= 2, "mode must be 0, 1 or 2");
+    PA_CHECK(mode >= 0 && mode <= 3, "mode must be 0, 1, 2 or 3");
+    PA_CHECK(mode != 3 || (B.born_m1 && B.imaging_period == 1),
+             "Born mode needs m1 and imaging_period 1");
     PA_CHECK(direction == 1 || direction == -1, "direction must be +1 or -1");
     PA_CHECK(slot_begin >= 0 && slot_end >= slot_begin, "bad slot range");
     PA_CHECK(B.tile_x >= 1 && (B.dim == 2 || B.tile_y >= 1), "tile sizes must be positive");
@@ -186,8 +188,8 @@ int pysit_amd_batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int 
     PA_CHECK(B.dim == 2 || (B.phiy[0] && B.phiy[1] && B.psi[0] && B.psi[1]),
              "3D state buffers missing");
     PA_CHECK(B.imaging_period >= 1, "imaging_period must be >= 1");
-    PA_CHECK(mode != 2 || B.ic, "adjoint mode needs the imaging accumulator");
-    PA_CHECK(mode == 0 || B.dw, "modes 1 and 2 need the dWaveOp store");
+    PA_CHECK(mode != 2 || !B.dw || B.ic, "imaging needs the FP64 accumulator");
+    PA_CHECK(mode == 0 || mode == 2 || B.dw, "modes 1 and 3 need the dWaveOp store");
     PA_CHECK(B.n_rec == 0 || (B.recs && B.rec_idx && B.rec_w), "receiver tables missing");
     PA_CHECK(seg_base >= 0 && seg_base <= slot_begin, "seg_base must not exceed slot_begin");
     if (active_tiles || active_recs) {

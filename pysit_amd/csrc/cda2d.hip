@@ -74,9 +74,12 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
     }
     const bool img = (MODE == 2) && (k % B.imaging_period == 0) && B.dw;
     const bool store = (MODE == 1) && (k % B.imaging_period == 0) && B.dw;
-    const T *dwi = img ? (const T *)B.dw + (long long)(slot - seg_base) * B.total_n + off : nullptr;
-    T *dwo = store ? (T *)B.dw + (long long)(slot - seg_base) * B.total_n + off : nullptr;
+    const long long soff = (long long)(slot - seg_base) * B.total_n + off;
+    const T *dwi = (img || MODE == 3) ? (const T *)B.dw + soff : nullptr;
+    T *dwo = store ? (T *)B.dw + soff : nullptr;
+    T *fst = B.fstore ? (T *)B.fstore + soff : nullptr;
     double *ic = (MODE == 2) ? B.ic + off : nullptr;
+    const T *m1 = (MODE == 3) ? (const T *)B.born_m1 : nullptr;
 
     T q[2 * R + 1], pq[2 * R + 1];
 #pragma unroll
@@ -130,9 +133,12 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
         }
         T f = (T)facc;
         if (P.rhs && zin) f += P.rhs[row + z];
+        const long long mrow = (long long)clampi(x - S.mx_off, S.mx_lo, S.mx_hi) * S.mnz;
+        // Born source: rhs_k = m1_pad * (-dWaveOp0_k)
+        if (MODE == 3 && zin) f += m1[mrow + mcol] * (-dwi[row + z]);
 
         const T um = zin ? uo[row + z] : T(0);
-        const T cv = model[(long long)clampi(x - S.mx_off, S.mx_lo, S.mx_hi) * S.mnz + mcol];
+        const T cv = model[mrow + mcol];
         wave_sync();
 
         // second derivatives: symmetric pairs, far taps first
@@ -199,6 +205,7 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
                 pzo[row + z] = pzn;
             }
             if (MODE == 1 && store) dwo[row + z] = dw;
+            if (fst) fst[row + z] = uc;
         }
         wave_sync();
 #pragma unroll
@@ -295,7 +302,8 @@ int run2d(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int 
         int rc;
         if (mode == 0) rc = launch2d_r<T, 0>(r, B, P, j, seg_base, nt, nr, st);
         else if (mode == 1) rc = launch2d_r<T, 1>(r, B, P, j, seg_base, nt, nr, st);
-        else rc = launch2d_r<T, 2>(r, B, P, j, seg_base, nt, 0, st);
+        else if (mode == 2) rc = launch2d_r<T, 2>(r, B, P, j, seg_base, nt, 0, st);
+        else rc = launch2d_r<T, 3>(r, B, P, j, seg_base, nt, nr, st);
         if (rc) return rc;
     }
     return PYSIT_AMD_OK;

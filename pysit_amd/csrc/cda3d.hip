@@ -90,9 +90,12 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
     }
     const bool img = (MODE == 2) && (k % B.imaging_period == 0) && B.dw;
     const bool store = (MODE == 1) && (k % B.imaging_period == 0) && B.dw;
-    const T *dwi = img ? (const T *)B.dw + (long long)(slot - seg_base) * B.total_n + off : nullptr;
-    T *dwo = store ? (T *)B.dw + (long long)(slot - seg_base) * B.total_n + off : nullptr;
+    const long long soff = (long long)(slot - seg_base) * B.total_n + off;
+    const T *dwi = (img || MODE == 3) ? (const T *)B.dw + soff : nullptr;
+    T *dwo = store ? (T *)B.dw + soff : nullptr;
+    T *fst = B.fstore ? (T *)B.fstore + soff : nullptr;
     double *ic = (MODE == 2) ? B.ic + off : nullptr;
+    const T *m1 = (MODE == 3) ? (const T *)B.born_m1 : nullptr;
 
     const long long yz = (long long)y * nz + z;
     T q[2 * R + 1], qx[2 * R + 1], qs[2 * R + 1];
@@ -182,8 +185,10 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
         }
         T f = (T)facc;
         if (P.rhs && valid) f += P.rhs[pl + yz];
+        const long long mi = (long long)clampi(x - S.mx_off, S.mx_lo, S.mx_hi) * S.mny * S.mnz + myz;
+        if (MODE == 3 && valid) f += m1[mi] * (-dwi[pl + yz]);
         const T um = valid ? uo[pl + yz] : T(0);
-        const T cv = model[(long long)clampi(x - S.mx_off, S.mx_lo, S.mx_hi) * S.mny * S.mnz + myz];
+        const T cv = model[mi];
         __syncthreads();
 
         if (valid) {
@@ -265,6 +270,7 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
                 pso[g] = psn;
             }
             if (MODE == 1 && store) dwo[g] = dw;
+            if (fst) fst[g] = uc;
         }
         __syncthreads();
 #pragma unroll
@@ -357,7 +363,8 @@ int run3d(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int 
         int rc;
         if (mode == 0) rc = launch3d_r<T, 0>(r, B, P, j, seg_base, nt, nr, st);
         else if (mode == 1) rc = launch3d_r<T, 1>(r, B, P, j, seg_base, nt, nr, st);
-        else rc = launch3d_r<T, 2>(r, B, P, j, seg_base, nt, 0, st);
+        else if (mode == 2) rc = launch3d_r<T, 2>(r, B, P, j, seg_base, nt, 0, st);
+        else rc = launch3d_r<T, 3>(r, B, P, j, seg_base, nt, nr, st);
         if (rc) return rc;
     }
     return PYSIT_AMD_OK;

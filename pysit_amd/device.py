@@ -431,13 +431,14 @@ class DeviceBatch:
             self.dw_slots = K
 
     # ----------------------------------------------------------------- forward
-    def forward(self, keep_dw=False, segment=None):
+    def forward(self, keep_dw=False, segment=None, store_field=False):
         """Run every shot forward from rest, recording traces.
 
         With ``keep_dw`` the state is prepared for :meth:`migrate`: either the
         whole dWaveOp history is stored (one segment) or checkpoints are kept
-        at segment boundaries.  Returns the per-shot traces (device FP64, in
-        the caller's shot order)."""
+        at segment boundaries.  ``store_field`` also keeps u_k of every step
+        (``field_store``).  Returns the per-shot traces (device FP64, in the
+        caller's shot order)."""
         for t in self.traces:
             t.zero_()
         for f in self.fstate.values():
@@ -445,19 +446,79 @@ class DeviceBatch:
         K = segment if segment else self.plan_segments(keep_dw)
         self.K = K
         self.ckpts = []
+        self.field_store = self._field_store() if store_field else None
         if keep_dw and K >= self.max_nt:
             self._ensure_dw(self.max_nt)
             b = self._batch(self.fstate, False, dw=self.dw)
+            b.fstore = self.field_store.data_ptr() if store_field else None
             self._run(b, 1, 0, self.max_nt, 1, 0)
         elif keep_dw:
+            if store_field:
+                raise ValueError("storing the wavefield needs the whole dWaveOp history in memory")
             b = self._batch(self.fstate, False)
             for j0 in range(0, self.max_nt, K):
                 self.ckpts.append({f: v.clone() for f, v in self.fstate.items()})
                 self._run(b, 0, j0, min(self.max_nt, j0 + K), 1, j0)
         else:
             b = self._batch(self.fstate, False)
+            b.fstore = self.field_store.data_ptr() if store_field else None
             self._run(b, 0, 0, self.max_nt, 1, 0)
         self.have_dw = keep_dw
+        return self.traces_in_caller_order()
+
+    def _field_store(self):
+        return torch.empty((self.max_nt, self.total_n), dtype=self.tdtype, device=self.dev)
+
+    def load_dw(self, per_shot_images):
+        """Use host-provided dWaveOp (per shot, caller order: a sequence of
+        padded arrays, one per imaged step) as the stored history."""
+        p = self.imaging_period
+        self._ensure_dw(self.max_nt)
+        self.dw.zero_()
+        for pos, orig in enumerate(self.order_idx):
+            s = self.shots[pos]
+            imgs = per_shot_images[orig]
+            for i, a in enumerate(imgs):
+                k = i * p
+                if k >= s.nt:
+                    break
+                slot = k + self.kshift[pos]
+                src = torch.as_tensor(np.ascontiguousarray(np.asarray(a).reshape(-1), dtype=self.dtype))
+                self.dw[slot, self.offs[pos]:self.offs[pos] + s.npts].copy_(src.to(self.dev))
+        self.K = self.max_nt
+        self.ckpts = []
+        self.have_dw = True
+
+    def shot_field(self, store, si_caller, k):
+        """Padded field of time level k of one shot from a field store."""
+        pos = self.order_idx.index(si_caller)
+        s = self.shots[pos]
+        slot = k + self.kshift[pos]
+        return store[slot, self.offs[pos]:self.offs[pos] + s.npts].view(s.shape)
+
+    def shot_dw(self, si_caller, k):
+        """Stored dWaveOp of step k of one shot (store-all histories only)."""
+        if self.ckpts:
+            raise RuntimeError("dWaveOp was checkpointed, not stored")
+        return self.shot_field(self.dw, si_caller, k)
+
+    def born(self, m1):
+        """Traces of the Born wavefield u1 driven by -m1 * dWaveOp0_k (mode 3);
+        needs the whole background dWaveOp history from forward(keep_dw=True).
+        ``m1`` has the model's shape.  Returns per-shot traces (caller order)."""
+        if not getattr(self, "have_dw", False) or self.ckpts or self.imaging_period != 1:
+            raise RuntimeError("born needs forward(keep_dw=True) with every step stored (imaging_period 1)")
+        m1 = torch.as_tensor(np.ascontiguousarray(m1, dtype=self.dtype)).to(self.dev)
+        if tuple(m1.shape) != tuple(self.model_shape):
+            raise ValueError(f"m1 shape {tuple(m1.shape)} != model shape {self.model_shape}")
+        for t in self.traces:
+            t.zero_()
+        state = self._new_state()
+        b = self._batch(state, False, dw=self.dw)
+        b.inj_ptr = b.inj_pos = b.inj_w = None  # u1 is driven by the Born source alone
+        b.born_m1 = m1.data_ptr()
+        self._born_keep = m1
+        self._run(b, 3, 0, self.max_nt, 1, 0)
         return self.traces_in_caller_order()
 
     def traces_in_caller_order(self):
@@ -467,11 +528,13 @@ class DeviceBatch:
         return out
 
     # ----------------------------------------------------------------- migrate
-    def migrate(self, residuals):
-        """Imaging condition -dt*p*sum_k q_k dWaveOp_k of every shot (padded,
-        FP64, concatenated at ``offs``).  ``residuals`` are per-shot (nt, nrec)
-        operands in the caller's shot order (numpy or device tensors)."""
-        if not getattr(self, "have_dw", False):
+    def migrate(self, residuals, store_field=False, image=True):
+        """Imaging condition sum_k q_k dWaveOp_k of every shot (padded, FP64,
+        concatenated at ``offs``; the caller applies -dt*p when folding).
+        ``residuals`` are per-shot (nt, nrec) operands in the caller's shot
+        order (numpy or device tensors).  ``store_field`` keeps q_k of every
+        step in ``adjoint_store``; ``image=False`` only propagates q."""
+        if image and not getattr(self, "have_dw", False):
             raise RuntimeError("migrate needs a forward(keep_dw=True) first")
         if self.adj_inj is None:
             raise RuntimeError("the shots carry no adjoint injection entries")
@@ -490,9 +553,20 @@ class DeviceBatch:
         for f in self.qstate.values():
             f.zero_()
         self.ic = torch.zeros(self.total_n, dtype=torch.float64, device=self.dev)
-        K = self.K
+        K = getattr(self, "K", self.max_nt)
+        if not hasattr(self, "ckpts"):
+            self.ckpts = []
         qb = self._batch(self.qstate, True, ic=self.ic, dw=None)
-        if not self.ckpts:
+        self.adjoint_store = None
+        if store_field:
+            self.adjoint_store = self._field_store()
+            qb.fstore = self.adjoint_store.data_ptr()
+            if self.ckpts and image:
+                raise ValueError("storing the adjoint field needs seg_base 0 (no checkpoints)")
+        if not image:
+            qb.dw = None
+            self._run(qb, 2, 0, self.max_nt, -1, 0, record=False)
+        elif not self.ckpts:
             qb.dw = self.dw.data_ptr()
             self._run(qb, 2, 0, self.max_nt, -1, 0, record=False)
         else:

@@ -1,0 +1,222 @@
+"""Time-domain modeling on the GPU with PySIT's ``TemporalModeling`` API.
+
+Mirrors pysit/modeling/temporal_modeling.py: ``forward_model`` (:60-169),
+``migrate_shot`` (:171-214), ``adjoint_model`` (:228-387) and
+``linear_forward_model`` (:389-546), with the same arguments and return
+dictionaries.  The per-step Python loops of the reference become one fused
+device loop per call (``pysit_amd.device.DeviceBatch``):
+
+* forward: traces are sampled from u_k before each step, rhs_k = S* w(k dt),
+  dWaveOp_k (the imaging factor) is produced by the step kernel itself;
+* adjoint: q runs the same step backwards in time driven by S* r_k, the FP64
+  imaging accumulator gathers q_k * dWaveOp_k for k % imaging_period == 0,
+  and the result is folded onto the physical grid with the factor
+  -dt * imaging_period;
+* Born: u1 is driven by -m1 * dWaveOp0_k.
+
+``forward_model(..., return_parameters=['dWaveOp'])`` returns a list of
+lightweight device handles instead of host arrays (one per imaged step).
+``TemporalLeastSquares`` copies that list and hands it back to
+``migrate_shot``, which then uses the device history (or its checkpoints)
+directly.  Indexing a handle with ``numpy.asarray`` downloads that step.
+
+Batched extensions run many shots in one launch per time step:
+``forward_model_batch`` and ``gradient_batch``.
+"""
+import numpy as np
+
+from ..core.receivers import adjoint_entries, sampling_entries
+from ..device import DeviceBatch, ShotGeometry
+
+__all__ = ["TemporalModeling", "DWaveOpHandle"]
+
+
+class DWaveOpHandle(object):
+    """dWaveOp of one imaged step of one shot, held on the device."""
+
+    __slots__ = ("batch", "shot", "k")
+
+    def __init__(self, batch, shot, k):
+        self.batch = batch
+        self.shot = shot
+        self.k = k
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.batch.shot_dw(self.shot, self.k).reshape(-1, 1).double().cpu().numpy()
+        return a if dtype is None else a.astype(dtype)
+
+
+def _handles(batch, shot, nsteps, p):
+    return [DWaveOpHandle(batch, shot, k) for k in range(0, nsteps, p)]
+
+
+class TemporalModeling(object):
+    """Forward, adjoint, migration and Born modeling for a time solver."""
+
+    solver_type = "time"
+    modeling_type = "time"
+
+    def __init__(self, solver, memory_budget=None, tile_x=None):
+        if self.solver_type != solver.supports["equation_dynamics"]:
+            raise TypeError(f"Argument 'solver' type {solver.supports['equation_dynamics']} does not match "
+                            f"modeling solver type {self.solver_type}.")
+        self.solver = solver
+        self.memory_budget = memory_budget
+        self.tile_x = tile_x
+
+    # ---------------------------------------------------------------- plumbing
+    def geometry(self, shot):
+        """The device description of ``shot`` for the current model."""
+        s = self.solver
+        shape = tuple(s.mesh.shape(include_bc=True, as_grid=True))
+        src, amp = shot.sources.injection(s.dt, s.nsteps)
+        rec = sampling_entries(shot.receivers, shape)
+        adj = adjoint_entries(shot.receivers)
+        return ShotGeometry(shape, s.kernel_pml(), s.model_window(), s.nsteps, src, amp, rec,
+                            shot.receivers.receiver_count, adj=adj)
+
+    def batch(self, shots, imaging_period=1):
+        s = self.solver
+        return DeviceBatch([self.geometry(sh) for sh in shots], s.unpadded_model(), s.mesh.deltas, s.dt,
+                           s.spatial_accuracy_order, dtype=s.dtype, imaging_period=imaging_period,
+                           memory_budget=self.memory_budget, tile_x=self.tile_x)
+
+    def _unpad_field(self, padded):
+        mesh = self.solver.mesh
+        return mesh.unpad_array(np.asarray(padded, dtype=np.float64).reshape(-1, 1), copy=True)
+
+    def _fold_images(self, batch, scale, shots_idx=None):
+        """Sum over shots of fold(scale * image) as an unpadded Perturbation."""
+        import torch
+        s = self.solver
+        mesh = s.mesh
+        out = torch.zeros(tuple(mesh.shape(as_grid=True)), dtype=torch.float64, device=batch.dev)
+        for i in (range(batch.n_shots) if shots_idx is None else shots_idx):
+            batch.fold(i, mesh.pad_widths(), out, scale=scale, accumulate=True)
+        return s.model_parameters.without_padding().perturbation(data=out.cpu().numpy().reshape(-1, 1))
+
+    # ----------------------------------------------------------------- forward
+    def forward_model(self, shot, m0, imaging_period=1, return_parameters=[]):
+        solver = self.solver
+        solver.model_parameters = m0
+        keep = "dWaveOp" in return_parameters
+        b = self.batch([shot], imaging_period)
+        traces = b.forward(keep_dw=keep, store_field="wavefield" in return_parameters)
+        retval = dict()
+        if "wavefield" in return_parameters:
+            retval["wavefield"] = [self._unpad_field(b.shot_field(b.field_store, 0, k).cpu().numpy())
+                                   for k in range(solver.nsteps)]
+            b.field_store = None
+        if keep:
+            retval["dWaveOp"] = _handles(b, 0, solver.nsteps, imaging_period)
+        if "simdata" in return_parameters:
+            retval["simdata"] = traces[0].cpu().numpy()
+        return retval
+
+    def forward_model_batch(self, shots, m0, imaging_period=1, keep_dw=False):
+        """Traces of many shots in one fused loop (host arrays, shot order).
+        With ``keep_dw`` the batch is kept in ``self.last_batch`` for
+        :meth:`gradient_batch`-style reuse."""
+        self.solver.model_parameters = m0
+        b = self.batch(shots, imaging_period)
+        traces = b.forward(keep_dw=keep_dw)
+        self.last_batch = b if keep_dw else None
+        return [t.cpu().numpy() for t in traces]
+
+    # ----------------------------------------------------------------- adjoint
+    def _batch_for(self, shot, dWaveOp, imaging_period):
+        """Reuse the device history behind handles, or upload host arrays."""
+        s = self.solver
+        if dWaveOp is not None and len(dWaveOp) and isinstance(dWaveOp[0], DWaveOpHandle):
+            h = dWaveOp[0]
+            b = h.batch
+            if (b.dt == s.dt and b.max_nt == s.nsteps and b.imaging_period == imaging_period
+                    and len(dWaveOp) == len(range(0, s.nsteps, imaging_period))):
+                return b, h.shot
+            raise ValueError("dWaveOp handles belong to another model, time axis or imaging period")
+        b = self.batch([shot], imaging_period)
+        if dWaveOp is not None:
+            b.load_dw([list(dWaveOp)])
+        return b, 0
+
+    def adjoint_model(self, shot, m0, operand_simdata, imaging_period=1, operand_dWaveOpAdj=None,
+                      operand_model=None, return_parameters=[], dWaveOp=None, wavefield=None):
+        solver = self.solver
+        solver.model_parameters = m0
+        if operand_dWaveOpAdj is not None or operand_model is not None:
+            raise NotImplementedError("the full-Hessian adjoint sources (operand_dWaveOpAdj) are not supported")
+        if "dWaveOpAdj" in return_parameters:
+            raise NotImplementedError("dWaveOpAdj is not returned by the device adjoint")
+        do_ic = dWaveOp is not None
+        if not do_ic and "imaging_condition" in return_parameters:
+            raise ValueError("To compute imaging condition, forward component must be specified.")
+        b, si = self._batch_for(shot, dWaveOp, imaging_period)
+        operands = [None] * b.n_shots
+        for i in range(b.n_shots):
+            operands[i] = np.zeros((b.shots[b.order_idx.index(i)].nt, b.shots[b.order_idx.index(i)].nrec))
+        operands[si] = np.asarray(operand_simdata, dtype=np.float64)
+        store = "adjointfield" in return_parameters
+        b.migrate(operands, store_field=store, image=do_ic)
+        retval = dict()
+        if store:
+            retval["adjointfield"] = [self._unpad_field(b.shot_field(b.adjoint_store, si, k).cpu().numpy())
+                                      for k in range(solver.nsteps)]
+            b.adjoint_store = None
+        if do_ic:
+            retval["imaging_condition"] = self._fold_images(b, -solver.dt * imaging_period, [si])
+        return retval
+
+    def migrate_shot(self, shot, m0, operand_simdata, imaging_period, operand_dWaveOpAdj=None,
+                     operand_model=None, dWaveOp=None, adjointfield=None, dWaveOpAdj=None,
+                     wavefield=None):
+        if dWaveOp is None:
+            dWaveOp = self.forward_model(shot, m0, imaging_period, return_parameters=["dWaveOp"])["dWaveOp"]
+        if dWaveOpAdj is not None:
+            raise NotImplementedError("dWaveOpAdj is not returned by the device adjoint")
+        rp = ["imaging_condition"]
+        if adjointfield is not None:
+            rp.append("adjointfield")
+        rv = self.adjoint_model(shot, m0, operand_simdata, imaging_period, operand_dWaveOpAdj, operand_model,
+                                return_parameters=rp, dWaveOp=dWaveOp, wavefield=wavefield)
+        if adjointfield is not None:
+            adjointfield[:] = rv["adjointfield"][:]
+        return rv["imaging_condition"].without_padding()
+
+    def gradient_batch(self, shots, m0, operands_fn, imaging_period=1):
+        """Forward all shots, form each shot's operand with ``operands_fn(i,
+        simdata)``, migrate them together and return (sum of migrate_shot
+        results, list of operands)."""
+        solver = self.solver
+        solver.model_parameters = m0
+        b = self.batch(shots, imaging_period)
+        traces = b.forward(keep_dw=True)
+        ops = [np.asarray(operands_fn(i, t.cpu().numpy()), dtype=np.float64) for i, t in enumerate(traces)]
+        b.migrate(ops)
+        g = self._fold_images(b, -solver.dt * imaging_period)
+        self.last_batch = b
+        return g, ops
+
+    # -------------------------------------------------------------------- Born
+    def linear_forward_model(self, shot, m0, m1, return_parameters=[], dWaveOp0=None):
+        solver = self.solver
+        solver.model_parameters = m0
+        if "dWaveOp1" in return_parameters:
+            raise NotImplementedError("dWaveOp1 is not returned by the device Born loop")
+        if dWaveOp0 is not None and len(dWaveOp0) and isinstance(dWaveOp0[0], DWaveOpHandle):
+            b, si = self._batch_for(shot, dWaveOp0, 1)
+        elif dWaveOp0 is not None:
+            b, si = self._batch_for(shot, dWaveOp0, 1)
+        else:
+            b = self.batch([shot], 1)
+            b.forward(keep_dw=True, segment=solver.nsteps)
+            si = 0
+        m1u = m1.without_padding().data.reshape(tuple(solver.mesh.shape(as_grid=True)))
+        traces = b.born(m1u)
+        retval = dict()
+        if "simdata" in return_parameters:
+            retval["simdata"] = traces[si].cpu().numpy()
+        if "dWaveOp0" in return_parameters:
+            retval["dWaveOp0"] = _handles(b, si, solver.nsteps, 1)
+        if "wavefield1" in return_parameters:
+            raise NotImplementedError("wavefield1 is not returned by the device Born loop")
+        return retval

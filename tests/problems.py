@@ -133,6 +133,48 @@ class Synthetic:
                             adj=(self.rec_pos, self.rec_chan, self.rec_w))
 
 
+def oracle_from_solver(solver, shot, order=None):
+    """The oracle Problem of one shot, built from the solver's host setup:
+    edge-padded C, the kernel PML arrays, the mesh sigma per axis and the
+    point-operator entries (flat padded indices)."""
+    from oracle.oracle import Problem
+    from pysit_amd.core.receivers import adjoint_entries, sampling_entries
+    mesh = solver.mesh
+    shape = tuple(mesh.shape(include_bc=True, as_grid=True))
+    oc = solver.operator_components
+    if mesh.dim == 2:
+        msig = [oc.sx.reshape(shape)[:, 0], oc.sz.reshape(shape)[0, :]]
+    else:
+        msig = [oc.sx.reshape(shape)[:, 0, 0], oc.sy.reshape(shape)[0, :, 0], oc.sz.reshape(shape)[0, 0, :]]
+    (spos, schan, sw), amp = shot.sources.injection(solver.dt, solver.nsteps)
+    src = (np.ravel_multi_index(tuple(spos.T), shape), schan, sw)
+    rec = sampling_entries(shot.receivers, shape)
+    apos, achan, aw = adjoint_entries(shot.receivers)
+    adj = (np.ravel_multi_index(tuple(apos.T), shape), achan, aw)
+    return Problem(shape, mesh.deltas, solver.dt, solver.nsteps, np.asarray(solver.model_parameters.C, dtype=np.float64),
+                   solver.kernel_pml(), msig, src, amp, rec, shot.receivers.receiver_count, adj=adj,
+                   order=order or solver.spatial_accuracy_order, pad=mesh.pad_widths())
+
+
+def c1_setup(order=4, precision="double", tf=1.5, nx=91, nz=71, pml=0.1):
+    """The horizontal-reflector case: x in [0, 0.9], z in [0, 0.7], dx = 0.01,
+    v = 1 + 0.5 exp(-((z - 0.45)/0.02)^2), PML(0.1, 100) on all sides, one
+    Ricker (10 Hz) source at depth 0.05 mid-x, receivers at every x node at
+    depth 0.05."""
+    import pysit_amd as pa
+    px = pa.PML(pml, 100) if pml else pa.Dirichlet()
+    d = pa.RectangularDomain((0.0, 0.9, px, px), (0.0, 0.7, px, px))
+    m = pa.CartesianMesh(d, nx, nz)
+    XX, ZZ = m.mesh_coords()
+    C = (1.0 + 0.5 * np.exp(-((ZZ - 0.45) / 0.02) ** 2)).reshape(-1, 1)
+    shots = pa.equispaced_acquisition(m, pa.RickerWavelet(10.0), sources=1, source_depth=0.05,
+                                      receivers="max", receiver_depth=0.05)
+    solver = pa.ConstantDensityAcousticWave(m, spatial_accuracy_order=order, trange=(0.0, tf),
+                                            kernel_implementation="hip", precision=precision)
+    model = solver.ModelParameters(m, {"C": C})
+    return m, shots, solver, model
+
+
 def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
