@@ -1,0 +1,239 @@
+#!/usr/bin/env python3
+"""Gradient throughput of the constant-density acoustic FWI path on one MI355X.
+
+Workload (the 2D layered case): 1001 x 301 nodes at dx = 10 m (1041 x 341
+padded with 20 PML cells per side), velocities 1500-4500 m/s in 12 seeded
+dipping layers times a smoothed (150 m) 5 % random field, 16 Ricker sources
+(8 Hz) at 20 m depth, 1001 receivers at 20 m, nt = 3000 at the CFL step,
+FP32, spatial order 8.  One "step" is one full misfit gradient of all 16
+shots: forward propagation recording traces and keeping dWaveOp, the misfit
+0.5*dt*||d - d_obs||^2, and the adjoint propagation with the FP64 imaging
+condition and the PML fold, through the torch.autograd.Function wrapper.
+
+value  = 2 * sum_shots(N_pad * nt) / seconds per gradient  (Gpt-updates/s),
+         recompute steps of a checkpoint schedule are not counted.
+plain  = the same gradient with one shot per launch (16 shot loops).
+roofline: the adjoint kernel (the dominant one): algorithmic bytes per
+         launch (FP32 fields: u_k, u_k-1, C read, u_k+1 written = 16 B per
+         node; imaging: dWaveOp read 4 B + FP64 accumulator read/write 16 B;
+         PML nodes +16 B for Phix, Phiz read/write) over its average launch
+         time from HIP events on the launch stream, against 8 TB/s.
+cpu_baseline: PySIT's own step kernel (cda_time_scalar_2D<float, 8>,
+         compiled here from the reference sources with the reference's flags
+         into oracle/_ref) stepping the same padded grid on the host cores,
+         one shot per thread; falls back to the oracle restatement.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "fwd+adjoint gradient Gpt-updates/s, % HBM roofline: batched-ckpt vs plain"
+
+
+def parse():
+    p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--shots", type=int, default=16)
+    p.add_argument("--nt", type=int, default=3000)
+    p.add_argument("--no-plain", action="store_true", help="skip the one-shot-per-launch comparison")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--cpu-seconds", type=float, default=6.0, help="target seconds per CPU baseline leg")
+    p.add_argument("--tile-x", type=int, default=None)
+    p.add_argument("--memory-budget-gb", type=float, default=None)
+    p.add_argument("--profile-run", action="store_true", help="minimal run for rocprofv3 (no plain, no CPU)")
+    return p.parse_args()
+
+
+def algorithmic_bytes(shapes_and_pml, itemsize):
+    """Per launch: forward and adjoint algorithmic bytes over all shots."""
+    fwd = adj = 0.0
+    for npts, npml in shapes_and_pml:
+        base = 4 * itemsize * npts
+        pml = 4 * itemsize * npml
+        fwd += base + pml
+        adj += base + pml + (itemsize + 16) * npts
+    return fwd, adj
+
+
+def pml_nodes(shape, pads):
+    inner = 1
+    for n, (l, r) in zip(shape, pads):
+        inner *= n - l - r
+    return int(np.prod(shape)) - inner
+
+
+def cpu_baseline(shape, pml_pairs, dt, deltas, seconds):
+    """Reference step kernel timed on the host cores, one shot per thread."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle as O
+    kind = "reference" if O.ref_available(fast=True) else "port"
+    use_ref = kind == "reference"
+    nx, nz = shape
+    n = nx * nz
+    rng = np.random.default_rng(0)
+    (xl, xr), (zl, zr) = pml_pairs
+
+    def run(nsteps):
+        f = [rng.standard_normal(n).astype(np.float32) * 1e-3 for _ in range(6)]
+        f[4] = np.full(n, 3000.0, np.float32)
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            px, pz, up = O.step2d(8, f[0], f[1], f[2], f[3], f[4], f[5], xl, xr, zl, zr, dt, deltas[0],
+                                  deltas[1], nx, nz, dtype=np.float32, use_ref=use_ref, fast=True)
+            f[0], f[3], f[1], f[2] = f[3], up, px, pz
+        return time.perf_counter() - t0
+
+    t_probe = run(3) / 3
+    steps = max(3, int(seconds / max(t_probe, 1e-6)))
+    t1 = run(steps)
+    single = n * steps / t1 / 1e9
+    threads = min(len(os.sched_getaffinity(0)), 16)
+    with ThreadPoolExecutor(threads) as ex:
+        t0 = time.perf_counter()
+        list(ex.map(run, [steps] * threads))
+        tm = time.perf_counter() - t0
+    multi = n * steps * threads / tm / 1e9
+    return {"value": multi, "unit": "Gpt-updates/s", "cores": threads, "kind": kind,
+            "single_thread": single,
+            "sample": (f"{steps} steps of cda_time_scalar_2D<float,8> on the {nx}x{nz} padded grid per thread, "
+                       f"{threads} threads (one shot each), -O3 -fopenmp -ffast-math; "
+                       f"single thread {single:.4f} Gpt/s; the step-kernel rate bounds the CPU gradient rate "
+                       f"from above (imaging and dWaveOp work not counted)")}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (shots shard across GPUs "
+              f"with one process per GPU, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import pysit_amd as pa
+    from pysit_amd.torch_fn import ShotProblem, ShotTraces
+    from pysit_amd.workloads import layered_2d
+
+    dev = torch.device("cuda", 0)
+    mesh, shots, solver, model = layered_2d(n_shots=args.shots, nt=args.nt)
+    shots = shots[:args.shots]
+    nx, nz = mesh.shape(as_grid=True)
+    shape = tuple(mesh.shape(include_bc=True, as_grid=True))
+    pads = mesh.pad_widths()
+    npts = int(np.prod(shape))
+    nt = solver.nsteps
+    dt = solver.dt
+    budget = int(args.memory_budget_gb * 2 ** 30) if args.memory_budget_gb else None
+
+    # observed data: the same model with a 2 % low-velocity lens (max C, so dt, unchanged)
+    X, Z = mesh.mesh_coords()
+    lens = 1.0 - 0.02 * np.exp(-((X - 5000.0) ** 2 + (Z - 1800.0) ** 2) / 300.0 ** 2)
+    true = solver.ModelParameters(mesh, {"C": model.C * lens.reshape(-1, 1)})
+    tm = pa.TemporalModeling(solver, memory_budget=budget, tile_x=args.tile_x)
+    d_obs = torch.from_numpy(np.stack(tm.forward_model_batch(shots, true))).to(dev)
+    tm.last_batch = None
+
+    m = torch.from_numpy((model.C.reshape(nx, nz) ** -2.0)).to(dev).requires_grad_(True)
+    problem = ShotProblem(solver, shots, memory_budget=budget, tile_x=args.tile_x)
+
+    def gradient(prob):
+        if m.grad is not None:
+            m.grad = None
+        tr = ShotTraces.apply(m, prob)
+        loss = 0.5 * dt * ((tr - d_obs[:tr.shape[0]]) ** 2).sum()
+        loss.backward()
+        return loss
+
+    for _ in range(args.warmup):
+        gradient(problem)
+    torch.cuda.synchronize()
+    problem.timer = {}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss = gradient(problem)
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / args.steps
+    timer, problem.timer = problem.timer, None
+    problem.batch.timer = None
+
+    updates = 2.0 * npts * nt * len(shots)
+    value = updates / t_step / 1e9
+    segment = problem.batch.K
+    gnorm = float(m.grad.norm())
+
+    # per-kernel average launch duration over the timed region (HIP events)
+    def avg_ms(mode):
+        tot = sum(s.elapsed_time(e) for s, e, _ in timer.get(mode, []))
+        n = sum(c for _, _, c in timer.get(mode, []))
+        return tot / max(n, 1), n
+
+    fwd_ms, n_fwd = avg_ms(1)
+    adj_ms, n_adj = avg_ms(2)
+    itemsize = 4 if solver.dtype == np.single else 8
+    npml = pml_nodes(shape, pads)
+    b_fwd, b_adj = algorithmic_bytes([(npts, npml)] * len(shots), itemsize)
+    ach_adj = b_adj / (adj_ms * 1e-3) / 1e9
+    ach_fwd = b_fwd / (fwd_ms * 1e-3) / 1e9
+    grad_bytes = nt * (b_fwd + b_adj)
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "Gpt-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded layered model, Ricker sources, observed data from a 2% lens model)",
+        "config": {"workload": "2D layered, 1001x301 (1041x341 padded), dx=10 m, order 8, PML 20, "
+                               f"{len(shots)} shots x 1001 receivers, nt={nt}, FP32, batched fused launches, "
+                               f"dWaveOp segment {segment} of {nt} slots",
+                   "grid_padded": list(shape), "shots": len(shots), "nt": nt, "order": 8,
+                   "tile_x": problem.batch.tile_x},
+        "roofline": {"bound": "hbm", "achieved": ach_adj, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach_adj / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_step2d<float,4,2> (adjoint + FP64 imaging)",
+                     "bytes_per_launch": b_adj, "avg_launch_us": adj_ms * 1e3, "launches": n_adj},
+        "forward_kernel": {"kernel": "k_step2d<float,4,1> (forward + dWaveOp store)",
+                           "achieved_gbs": ach_fwd, "frac": ach_fwd / HBM_PEAK_GBS,
+                           "avg_launch_us": fwd_ms * 1e3, "bytes_per_launch": b_fwd, "launches": n_fwd},
+        "gradient_roofline_frac": grad_bytes / t_step / 1e9 / HBM_PEAK_GBS,
+        "bytes_per_update": grad_bytes / updates,
+        "misfit": float(loss), "grad_norm": gnorm,
+    }
+
+    if not (args.no_plain or args.profile_run):
+        plain = [ShotProblem(solver, [s], memory_budget=budget, tile_x=args.tile_x) for s in shots]
+        for p in plain[:1]:
+            gradient(p)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for p in plain:
+            gradient(p)
+            p.batch = None
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t0
+        result["plain"] = {"value": updates / t_plain / 1e9, "ms_per_gradient": t_plain * 1e3,
+                           "form": "one shot per launch, same kernels, 16 sequential shot gradients"}
+        result["batched_vs_plain"] = t_plain / t_step
+
+    if not (args.no_cpu or args.profile_run):
+        kp = solver.kernel_pml()
+        result["cpu_baseline"] = cpu_baseline(shape, kp, dt, mesh.deltas, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -139,13 +139,19 @@ typedef struct {
      * sorted so that, for every slot, the active tiles form a prefix.        */
     int n_tiles;
     const int *tiles;
-    /* injection entries per tile, CSR: entries [inj_ptr[t], inj_ptr[t+1])
-     * of tile t, sorted by x (then y); inj_pos = {x, y, z, chan}           */
+    /* injection, two-level CSR: tile t owns destination nodes
+     * [inj_ptr[t], inj_ptr[t+1]); node n sits at local position inj_node[n]
+     * (2D: row_in_tile*64 + z - z0; 3D: (row_in_tile*8 + y - y0)*64 + z - z0)
+     * and sums entries [inj_nptr[n], inj_nptr[n+1]) in order:
+     * rhs += inj_w[e] * amp[k * namp + inj_chan[e]]                        */
     const int *inj_ptr;
-    const int *inj_pos;
+    const int *inj_node;
+    const int *inj_nptr;
+    const int *inj_chan;
     const double *inj_w;
     /* receiver sampling: rec {shot, r, e0, e1}; entries rec_idx (flat index
-     * in the shot's padded grid) and rec_w (FP64, weight * prod(deltas))    */
+     * in the shot's padded grid) and rec_w (FP64, weight * prod(deltas));
+     * 16 lanes share one receiver and reduce in a fixed order              */
     int n_rec;
     const int *recs;
     const long long *rec_idx;

@@ -39,7 +39,7 @@ class Batch(ctypes.Structure):
         ("dt", _d), ("dx", _d), ("dy", _d), ("dz", _d),
         ("n_shots", _i), ("shots", _vp),
         ("n_tiles", _i), ("tiles", _vp),
-        ("inj_ptr", _vp), ("inj_pos", _vp), ("inj_w", _vp),
+        ("inj_ptr", _vp), ("inj_node", _vp), ("inj_nptr", _vp), ("inj_chan", _vp), ("inj_w", _vp),
         ("n_rec", _i), ("recs", _vp), ("rec_idx", _vp), ("rec_w", _vp),
         ("u", _vp * 2), ("phix", _vp * 2), ("phiy", _vp * 2), ("phiz", _vp * 2), ("psi", _vp * 2),
         ("total_n", _ll),

@@ -23,6 +23,7 @@
 namespace pa {
 
 constexpr int TY3 = 8;  // y rows per workgroup (one wavefront each)
+constexpr int TX3 = 8;  // x planes per tile
 
 template <typename T>
 struct Step3DParams {
@@ -44,7 +45,7 @@ struct Planes3D {
 template <typename T, int R, int MODE>
 __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DParams<T> &P,
                                        const pysit_amd_shot &S, int tile, int flags, int tloc,
-                                       int slot, int seg_base, Planes3D<T, R> &L)
+                                       int slot, int seg_base, Planes3D<T, R> &L, T *lds_f)
 {
     const int k = slot - S.kshift;
     if (k < 0 || k >= S.nt) return;
@@ -57,7 +58,7 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
     const int tx = tloc / (S.ntz * S.nty);
     const int z0 = tz * 64, y0 = tyi * TY3;
     const int z = z0 + lane, y = y0 + ty;
-    const int x0 = tx * B.tile_x, x1 = min(x0 + B.tile_x, nx);
+    const int x0 = tx * TX3, x1 = min(x0 + TX3, nx);
     const bool valid = (z < nz) && (y < ny);
     const bool full = flags & 1;
     const int taken = (MODE == 2) ? (S.nt - 1 - k) : k;
@@ -109,14 +110,25 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
     }
     q[2 * R] = qx[2 * R] = qs[2 * R] = T(0);
 
-    int e = 0, e_end = 0;
-    const int4 *inj = nullptr;
+    // sources of this tile, gathered into LDS (x-plane, y row, z lane): one
+    // thread per destination node, entries summed in the reference's order
+    bool has_inj = false;
     if (B.inj_ptr) {
-        e = B.inj_ptr[tile];
-        e_end = B.inj_ptr[tile + 1];
-        inj = reinterpret_cast<const int4 *>(B.inj_pos);
+        const int n0 = B.inj_ptr[tile], n1 = B.inj_ptr[tile + 1];
+        if (n1 > n0) {
+            const double *amp_k = S.amp + (long long)k * S.namp;
+            for (int i = tid; i < TX3 * TY3 * 64; i += TY3 * 64) lds_f[i] = T(0);
+            __syncthreads();
+            for (int n = n0 + tid; n < n1; n += TY3 * 64) {
+                double acc = 0.0;
+                for (int e = B.inj_nptr[n]; e < B.inj_nptr[n + 1]; ++e)
+                    acc += B.inj_w[e] * amp_k[B.inj_chan[e]];
+                lds_f[B.inj_node[n]] = (T)acc;
+            }
+            __syncthreads();
+            has_inj = true;
+        }
     }
-    const double *amp_k = S.amp ? S.amp + (long long)k * S.namp : nullptr;
 
     // halo elements of an LY x LZ plane outside the TY3 x 64 centre
     constexpr int HU = 2 * R * LZ + TY3 * 2 * R;
@@ -177,13 +189,7 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
                 L.pz[rr][lz] = in ? pz[pl + (long long)gy * nz + gz] : T(0);
             }
         }
-        double facc = 0.0;
-        while (e < e_end && inj[e].x == x) {
-            const int4 en = inj[e];
-            if (en.y == y && en.z == z) facc += B.inj_w[e] * amp_k[en.w];
-            ++e;
-        }
-        T f = (T)facc;
+        T f = has_inj ? lds_f[((x - x0) * TY3 + ty) * 64 + lane] : T(0);
         if (P.rhs && valid) f += P.rhs[pl + yz];
         const long long mi = (long long)clampi(x - S.mx_off, S.mx_lo, S.mx_hi) * S.mny * S.mnz + myz;
         if (MODE == 3 && valid) f += m1[mi] * (-dwi[pl + yz]);
@@ -282,17 +288,25 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
     }
 }
 
+// Receiver sampling, 16 lanes per receiver with a fixed reduction order
+// (as in 2D): entries g, g+16, ... per lane, then a butterfly.
 template <typename T>
-__device__ __forceinline__ void sample3d(const pysit_amd_batch &B, int slot, int rl)
+__device__ __forceinline__ void sample_group3(const pysit_amd_batch &B, int slot, int rec, int g)
 {
-    const int4 rd = reinterpret_cast<const int4 *>(B.recs)[rl];
-    const pysit_amd_shot &S = B.shots[rd.x];
-    const int k = slot - S.kshift;
-    if (k < 0 || k >= S.nt) return;
-    const T *u = (const T *)B.u[k & 1] + S.off;
+    const bool act = rec < B.n_rec;
+    int4 rd = make_int4(0, 0, 0, 0);
+    if (act) rd = reinterpret_cast<const int4 *>(B.recs)[rec];
+    const pysit_amd_shot *S = act ? &B.shots[rd.x] : nullptr;
+    const int k = act ? slot - S->kshift : -1;
+    const bool live = act && k >= 0 && k < S->nt;
     double acc = 0.0;
-    for (int e = rd.z; e < rd.w; ++e) acc += B.rec_w[e] * (double)u[B.rec_idx[e]];
-    S.trace[(long long)k * S.nrec + rd.y] = acc;
+    if (live) {
+        const T *u = (const T *)B.u[k & 1] + S->off;
+        for (int e = rd.z + g; e < rd.w; e += 16) acc += B.rec_w[e] * (double)u[B.rec_idx[e]];
+    }
+#pragma unroll
+    for (int m = 8; m >= 1; m >>= 1) acc += __shfl_xor(acc, m, 16);
+    if (live && g == 0) S->trace[(long long)k * S->nrec + rd.y] = acc;
 }
 
 template <typename T, int R, int MODE>
@@ -301,19 +315,20 @@ __global__ __launch_bounds__(512) void k_step3d(const pysit_amd_batch B, const S
                                                 int n_rec_lanes)
 {
     __shared__ Planes3D<T, R> L;
+    __shared__ T lds_f[TX3 * TY3 * 64];
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     if (blk < n_tile_blocks) {
         if (P.single) {
-            tile3d<T, R, MODE>(B, P, P.shot0, blk, 1, blk, slot, seg_base, L);
+            tile3d<T, R, MODE>(B, P, P.shot0, blk, 1, blk, slot, seg_base, L, lds_f);
         } else {
             const int4 td = reinterpret_cast<const int4 *>(B.tiles)[blk];
-            tile3d<T, R, MODE>(B, P, B.shots[td.x], blk, td.z, td.y, slot, seg_base, L);
+            tile3d<T, R, MODE>(B, P, B.shots[td.x], blk, td.z, td.y, slot, seg_base, L, lds_f);
         }
         return;
     }
     if (MODE != 2) {
-        const int rl = (blk - n_tile_blocks) * (TY3 * 64) + threadIdx.x;
-        if (rl < n_rec_lanes) sample3d<T>(B, slot, rl);
+        const int rec = (blk - n_tile_blocks) * (TY3 * 4) + (threadIdx.x >> 4);
+        sample_group3<T>(B, slot, rec < n_rec_lanes ? rec : B.n_rec, threadIdx.x & 15);
     }
 }
 
@@ -321,7 +336,7 @@ template <typename T, int R, int MODE>
 static int launch3d(const pysit_amd_batch &B, const Step3DParams<T> &P, int slot, int seg_base,
                     int n_tiles, int n_recs, hipStream_t st)
 {
-    const int rec_blocks = (MODE != 2) ? (n_recs + TY3 * 64 - 1) / (TY3 * 64) : 0;
+    const int rec_blocks = (MODE != 2) ? (n_recs + TY3 * 4 - 1) / (TY3 * 4) : 0;
     const int blocks = n_tiles + rec_blocks;
     if (blocks == 0) return PYSIT_AMD_OK;
     hipLaunchKernelGGL((k_step3d<T, R, MODE>), dim3(blocks), dim3(TY3 * 64), 0, st, B, P, slot,
@@ -348,6 +363,7 @@ template <typename T>
 int run3d(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int direction,
           int seg_base, const int *active_tiles, const int *active_recs, hipStream_t st)
 {
+    PA_CHECK(B.tile_x == TX3 && B.tile_y == TY3, "3D tiles are 8 x 8 (tile_x, tile_y)");
     Step3DParams<T> P{};
     const int r = B.order / 2;
     P.cx = axis_coef<T>(r, B.dx);
@@ -403,7 +419,7 @@ int step3d_entry(int order, const T *km1_u, const T *k_phix, const T *k_phiy, co
     B.dtype = sizeof(T) == 8;
     B.dim = 3;
     B.order = order;
-    B.tile_x = 8;
+    B.tile_x = TX3;
     B.tile_y = TY3;
     B.dt = dt;
     B.dx = dx;
@@ -437,7 +453,7 @@ int step3d_entry(int order, const T *km1_u, const T *k_phix, const T *k_phiy, co
     S.kshift = 0;
     S.ntz = (nz + 63) / 64;
     S.nty = (ny + TY3 - 1) / TY3;
-    S.ntx = (nx + B.tile_x - 1) / B.tile_x;
+    S.ntx = (nx + TX3 - 1) / TX3;
     S.off = 0;
     S.model = C;
     S.mny = ny;

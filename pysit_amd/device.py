@@ -117,14 +117,25 @@ class DeviceBatch:
         self.deltas = tuple(float(d) for d in deltas)
         self.dt = float(dt)
         self.imaging_period = int(imaging_period)
-        self.tile_x = int(tile_x) if tile_x else (16 if self.dim == 2 else 8)
+        if self.dim == 2:
+            # the 2D kernel's register blocking fixes the tile rows per dtype
+            need = 16 if self.dtype == np.float32 else 8
+            if tile_x and int(tile_x) != need:
+                raise ValueError(f"2D tiles are {need} rows along x for {self.dtype}")
+            self.tile_x = need
+        else:
+            self.tile_x = int(tile_x) if tile_x else 8
         self.memory_budget = memory_budget
 
-        model = np.ascontiguousarray(model, dtype=self.dtype)
-        if model.ndim != self.dim:
+        if isinstance(model, torch.Tensor):
+            self.model = model.detach().to(device=self.dev, dtype=self.tdtype).contiguous()
+        else:
+            self.model = torch.from_numpy(np.ascontiguousarray(model, dtype=self.dtype)).to(self.dev)
+        if self.model.dim() != self.dim:
             raise ValueError("model must have the grid's dimension")
-        self.model_shape = model.shape
-        self.model = torch.from_numpy(model).to(self.dev)
+        self.model_shape = tuple(self.model.shape)
+        # optional per-mode launch timing: {mode: [(start, end, n_launches)]}
+        self.timer = None
 
         self.max_nt = max(s.nt for s in shots)
         kshift = [self.max_nt - s.nt for s in shots]
@@ -263,8 +274,10 @@ class DeviceBatch:
         self.shots_adj = None
 
     def _injection_tables(self, entries):
-        ptr_counts = np.zeros(self.n_tiles, dtype=np.int64)
-        pos_all, w_all, tile_all = [], [], []
+        """Two-level CSR: tiles -> destination nodes -> entries.  Entries of
+        one node keep the order they were given in (the order the reference's
+        sparse product accumulates in)."""
+        node_tile, node_pos, node_key, e_chan, e_w = [], [], [], [], []
         for si, (s, e) in enumerate(zip(self.shots, entries)):
             pos, chan, w = e
             if pos.shape[0] == 0:
@@ -274,28 +287,36 @@ class DeviceBatch:
             tloc = self._tile_of_pos(self.tiles_info[si][0], pos)
             tglob = np.array([self._tile_index[(si, int(t))] for t in tloc], dtype=np.int64)
             if self.dim == 2:
-                p4 = np.stack([pos[:, 0], np.zeros_like(pos[:, 0]), pos[:, 1], chan], axis=1)
+                local = (pos[:, 0] % self.tile_x) * TILE_Z + pos[:, 1] % TILE_Z
             else:
-                p4 = np.stack([pos[:, 0], pos[:, 1], pos[:, 2], chan], axis=1)
-            # within a tile: by x, keeping the reference's accumulation order
-            key = np.lexsort((np.arange(pos.shape[0]), pos[:, 0], tglob))
-            pos_all.append(p4[key])
-            w_all.append(w[key])
-            tile_all.append(tglob[key])
-        if pos_all:
-            tile_all = np.concatenate(tile_all)
-            key = np.argsort(tile_all, kind="stable")
-            pos_all = np.concatenate(pos_all)[key].astype(np.int32)
-            w_all = np.concatenate(w_all)[key]
-            tile_all = tile_all[key]
-            ptr_counts = np.bincount(tile_all, minlength=self.n_tiles)
+                local = ((pos[:, 0] % self.tile_x) * TILE_Y3 + pos[:, 1] % TILE_Y3) * TILE_Z + pos[:, 2] % TILE_Z
+            node_tile.append(tglob)
+            node_pos.append(local)
+            e_chan.append(chan)
+            e_w.append(w)
+        if node_tile:
+            tg = np.concatenate(node_tile)
+            lp = np.concatenate(node_pos)
+            ch = np.concatenate(e_chan)
+            wt = np.concatenate(e_w)
+            order = np.lexsort((np.arange(tg.size), lp, tg))  # stable within a node
+            tg, lp, ch, wt = tg[order], lp[order], ch[order], wt[order]
+            key = tg * (1 << 20) + lp
+            starts = np.concatenate([[True], key[1:] != key[:-1]])
+            node_first = np.nonzero(starts)[0]
+            n_tile = tg[node_first]
+            n_pos = lp[node_first].astype(np.int32)
+            nptr = np.concatenate([node_first, [tg.size]]).astype(np.int32)
+            tcount = np.bincount(n_tile, minlength=self.n_tiles)
         else:
-            pos_all = np.zeros((0, 4), np.int32)
-            w_all = np.zeros(0)
-        ptr = np.concatenate([[0], np.cumsum(ptr_counts)]).astype(np.int32)
-        return (torch.from_numpy(ptr).to(self.dev),
-                torch.from_numpy(np.ascontiguousarray(pos_all).reshape(-1)).to(self.dev),
-                torch.from_numpy(np.ascontiguousarray(w_all, dtype=np.float64)).to(self.dev))
+            n_pos = np.zeros(0, np.int32)
+            nptr = np.zeros(1, np.int32)
+            ch = np.zeros(0, np.int64)
+            wt = np.zeros(0)
+            tcount = np.zeros(self.n_tiles, np.int64)
+        ptr = np.concatenate([[0], np.cumsum(tcount)]).astype(np.int32)
+        t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(self.dev)
+        return (t(ptr, np.int32), t(n_pos, np.int32), t(nptr, np.int32), t(ch, np.int32), t(wt, np.float64))
 
     def _shot_table(self, amps, namps):
         arr = (N.Shot * self.n_shots)()
@@ -370,7 +391,7 @@ class DeviceBatch:
         b.n_tiles = self.n_tiles
         b.tiles = self.tiles.data_ptr()
         inj = self.adj_inj if adjoint else self.fwd_inj
-        b.inj_ptr, b.inj_pos, b.inj_w = inj[0].data_ptr(), inj[1].data_ptr(), inj[2].data_ptr()
+        b.inj_ptr, b.inj_node, b.inj_nptr, b.inj_chan, b.inj_w = [x.data_ptr() for x in inj]
         b.n_rec = 0 if adjoint else self.n_rec
         b.recs = self.recs.data_ptr() if self.n_rec else None
         b.rec_idx = self.rec_idx.data_ptr() if self.n_rec else None
@@ -389,8 +410,15 @@ class DeviceBatch:
     def _run(self, b, mode, j0, j1, direction, seg_base, record=True):
         at = np.ascontiguousarray(self.active_tiles)
         ar = np.ascontiguousarray(self.active_recs if record else np.zeros_like(self.active_recs))
+        ev = None
+        if self.timer is not None:
+            ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            ev[0].record()
         N.call("pysit_amd_batch_run", ctypes.byref(b), mode, j0, j1, direction, seg_base,
                at.ctypes.data_as(ctypes.c_void_p), ar.ctypes.data_as(ctypes.c_void_p), _stream())
+        if ev is not None:
+            ev[1].record()
+            self.timer.setdefault(mode, []).append((ev[0], ev[1], j1 - j0))
 
     # ---------------------------------------------------------- checkpointing
     def plan_segments(self, keep_dw):
@@ -515,7 +543,7 @@ class DeviceBatch:
             t.zero_()
         state = self._new_state()
         b = self._batch(state, False, dw=self.dw)
-        b.inj_ptr = b.inj_pos = b.inj_w = None  # u1 is driven by the Born source alone
+        b.inj_ptr = b.inj_node = b.inj_nptr = b.inj_chan = b.inj_w = None  # u1: Born source alone
         b.born_m1 = m1.data_ptr()
         self._born_keep = m1
         self._run(b, 3, 0, self.max_nt, 1, 0)

@@ -1,0 +1,92 @@
+"""``torch.autograd`` wrapper: traces of many shots as a function of the model.
+
+``ShotTraces.apply(m, problem)`` returns the recorded traces
+``(n_shots, nt, nrec)`` (FP64) of every shot in ``problem`` for the slowness
+squared ``m = C^-2`` on the unpadded model grid.  Its backward is the HIP
+adjoint: the upstream trace gradient G is injected at the receivers
+(S*, no prod(deltas)), propagated backwards in time, and the zero-lag imaging
+condition against the stored (or checkpoint-recomputed) dWaveOp gives
+
+    dL/dm = -prod(deltas) * fold( sum_k q_k[G] * dWaveOp_k ),
+
+the transpose of the Born operator of
+pysit/modeling/temporal_modeling.py:389-546 with PySIT's adjoint
+(:228-387): exact without PML; with PML it carries the reference's forward-PML
+adjoint, i.e. it equals PySIT's migrate_shot up to the factor
+-prod(deltas)/dt (``TemporalLeastSquares`` gradients are reproduced by
+``L = 0.5 * dt * ||d - d_obs||^2``).  dt is fixed by the model given to the
+forward (CFL), as in PySIT.
+"""
+import numpy as np
+import torch
+
+from .device import DeviceBatch
+
+__all__ = ["ShotProblem", "ShotTraces", "traces"]
+
+
+class ShotProblem(object):
+    """Shots of one solver / mesh for the autograd path (geometry built once
+    per model change by ``TemporalModeling.geometry``)."""
+
+    def __init__(self, solver, shots, imaging_period=1, memory_budget=None, tile_x=None, segment=None):
+        from .modeling.temporal_modeling import TemporalModeling
+        self.solver = solver
+        self.shots = shots
+        self.tm = TemporalModeling(solver, memory_budget=memory_budget, tile_x=tile_x)
+        self.imaging_period = imaging_period
+        self.segment = segment
+        self.batch = None
+        self._key = None
+        self.timer = None
+
+    def prepare(self, m_host):
+        s = self.solver
+        C = np.sqrt(1.0 / np.asarray(m_host, dtype=np.float64)).reshape(-1, 1)
+        s.model_parameters = s.ModelParameters(s.mesh, {"C": C})
+        return s
+
+    def device_batch(self, m):
+        """The batch for slowness squared ``m`` (device tensor): rebuilt when
+        dt or the step count change, else the model is updated in place."""
+        s = self.prepare(m.detach().cpu().numpy())
+        key = (s.dt, s.nsteps)
+        C = (m.detach() ** -0.5).to(torch.float64)
+        if self.batch is None or self._key != key:
+            self.batch = None
+            self.batch = DeviceBatch([self.tm.geometry(sh) for sh in self.shots], C, s.mesh.deltas, s.dt,
+                                     s.spatial_accuracy_order, dtype=s.dtype, imaging_period=self.imaging_period,
+                                     memory_budget=self.tm.memory_budget, tile_x=self.tm.tile_x)
+            self._key = key
+        else:
+            self.batch.model.copy_(C.reshape(self.batch.model.shape))
+        self.batch.timer = self.timer
+        return self.batch
+
+
+class ShotTraces(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, m, problem):
+        b = problem.device_batch(m)
+        tr = b.forward(keep_dw=True, segment=problem.segment)
+        ctx.problem = problem
+        ctx.batch = b
+        ctx.m_shape = m.shape
+        return torch.stack(tr)
+
+    @staticmethod
+    def backward(ctx, grad_traces):
+        b = ctx.batch
+        p = ctx.problem
+        mesh = p.solver.mesh
+        b.migrate([grad_traces[i].contiguous() for i in range(b.n_shots)])
+        out = torch.zeros(tuple(mesh.shape(as_grid=True)), dtype=torch.float64, device=b.dev)
+        scale = -float(np.prod(mesh.deltas)) * p.imaging_period
+        for i in range(b.n_shots):
+            b.fold(i, mesh.pad_widths(), out, scale=scale, accumulate=True)
+        return out.reshape(ctx.m_shape).to(grad_traces.dtype), None
+
+
+def traces(m, problem):
+    """Recorded traces (n_shots, nt, nrec) of ``problem`` for slowness squared ``m``."""
+    return ShotTraces.apply(m, problem)

@@ -1,0 +1,136 @@
+"""Synthetic benchmark and test configurations.
+
+Each builder returns ``(mesh, shots, solver, model)`` (plus extras) in the
+package's own API, seeded so that every run sees the same numbers:
+
+* ``horizontal_reflector_2d`` -- the reference's CPU case: 91 x 71 nodes on
+  [0, 0.9] x [0, 0.7], v = 1 + 0.5 exp(-((z-0.45)/0.02)^2), PML(0.1, 100),
+  one 10 Hz Ricker at depth 0.05, 91 receivers, T = 1.5.
+* ``layered_2d`` -- nz x nx with spacing dx, velocities 1500..vmax in
+  ``n_layers`` dipping layers (seeded dips), times (1 + 0.05 g) with g a
+  Gaussian random field smoothed with sigma = 150 m and scaled to unit RMS,
+  clipped to [vmin, vmax]; PML of ``pml_cells`` nodes.
+* ``layered_3d`` -- the 3D analogue with a Gaussian low-velocity anomaly.
+"""
+import numpy as np
+from scipy.ndimage import gaussian_filter
+
+from .core import (RectangularDomain, PML, Dirichlet, CartesianMesh, RickerWavelet, PointSource, PointReceiver,
+                   ReceiverSet, Shot, equispaced_acquisition)
+from .solvers import ConstantDensityAcousticWave
+
+__all__ = ["horizontal_reflector_2d", "layered_velocity_2d", "layered_2d", "layered_velocity_3d", "layered_3d",
+           "pml_amplitude", "trange_for_steps"]
+
+
+def pml_amplitude(vmax, width, reflection=1e-3):
+    """Quadratic-profile damping amplitude for a target normal-incidence
+    reflection: 3 vmax ln(1/R) / (2 width)."""
+    return 3.0 * vmax * np.log(1.0 / reflection) / (2.0 * width)
+
+
+def trange_for_steps(dt, nt):
+    """A time range whose ceil((tf - t0)/dt) is exactly nt."""
+    return (0.0, (nt - 0.5) * dt)
+
+
+def horizontal_reflector_2d(order=4, precision="double", tf=1.5):
+    p = PML(0.1, 100)
+    d = RectangularDomain((0.0, 0.9, p, p), (0.0, 0.7, p, p))
+    m = CartesianMesh(d, 91, 71)
+    X, Z = m.mesh_coords()
+    C = (1.0 + 0.5 * np.exp(-((Z - 0.45) / 0.02) ** 2)).reshape(-1, 1)
+    shots = equispaced_acquisition(m, RickerWavelet(10.0), sources=1, source_depth=0.05, receivers="max",
+                                   receiver_depth=0.05)
+    solver = ConstantDensityAcousticWave(m, spatial_accuracy_order=order, trange=(0.0, tf),
+                                         kernel_implementation="hip", precision=precision)
+    return m, shots, solver, solver.ModelParameters(m, {"C": C})
+
+
+def layered_velocity_2d(nx, nz, dx, n_layers=12, vmin=1500.0, vmax=4500.0, seed=0, field_sigma=150.0,
+                        field_amp=0.05):
+    """(nx, nz) velocity: dipping layers plus a smoothed random field."""
+    rng = np.random.default_rng(seed)
+    x = np.arange(nx) * dx
+    z = np.arange(nz) * dx
+    depth = z[-1]
+    tops = np.sort(rng.uniform(0.05 * depth, 0.95 * depth, n_layers - 1))
+    dips = rng.uniform(-0.12, 0.12, n_layers - 1)  # slopes (about +-7 degrees)
+    xc = x.mean()
+    vel_layers = np.linspace(vmin, vmax, n_layers)
+    v = np.full((nx, nz), vel_layers[0])
+    for i in range(n_layers - 1):
+        iface = tops[i] + dips[i] * (x - xc)
+        below = z[None, :] >= iface[:, None]
+        v[below] = vel_layers[i + 1]
+    g = gaussian_filter(rng.standard_normal((nx, nz)), sigma=field_sigma / dx, mode="reflect")
+    g /= np.sqrt(np.mean(g ** 2))
+    v = v * (1.0 + field_amp * g)
+    return np.clip(v, vmin, vmax)
+
+
+def layered_2d(nx=1001, nz=301, dx=10.0, n_shots=16, nt=3000, order=8, precision="single", pml_cells=20,
+               f0=8.0, depth=20.0, seed=0, n_receivers=None, model=None):
+    """The 2D layered case: ``n_shots`` Ricker sources evenly spaced at
+    ``depth``, one receiver per surface node at ``depth``, ``nt`` steps at the
+    CFL dt of the model."""
+    v = layered_velocity_2d(nx, nz, dx, seed=seed) if model is None else model
+    width = pml_cells * dx
+    A = pml_amplitude(v.max(), width)
+    p = PML(width, A)
+    d = RectangularDomain((0.0, (nx - 1) * dx, p, p), (0.0, (nz - 1) * dx, p, p))
+    m = CartesianMesh(d, nx, nz)
+    shots = equispaced_acquisition(m, RickerWavelet(f0), sources=n_shots, source_depth=depth,
+                                   receivers=n_receivers or nx, receiver_depth=depth)
+    dt = (1.0 / 6.0) * dx / v.max()
+    solver = ConstantDensityAcousticWave(m, spatial_accuracy_order=order, trange=trange_for_steps(dt, nt),
+                                         kernel_implementation="hip", precision=precision)
+    model = solver.ModelParameters(m, {"C": v.reshape(-1, 1)})
+    solver.model_parameters = model
+    assert solver.nsteps == nt, (solver.nsteps, nt)
+    return m, shots, solver, model
+
+
+def layered_velocity_3d(n, dx, vmin=1500.0, vmax=4000.0, n_layers=8, seed=0, anomaly_radius=400.0,
+                        anomaly=-0.10):
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = n
+    z = np.arange(nz) * dx
+    tops = np.sort(rng.uniform(0.05 * z[-1], 0.95 * z[-1], n_layers - 1))
+    vel = np.linspace(vmin, vmax, n_layers)
+    v1 = np.full(nz, vel[0])
+    for i in range(n_layers - 1):
+        v1[z >= tops[i]] = vel[i + 1]
+    v = np.broadcast_to(v1[None, None, :], (nx, ny, nz)).copy()
+    X, Y, Z = np.meshgrid(np.arange(nx) * dx, np.arange(ny) * dx, z, indexing="ij")
+    c = (X.mean(), Y.mean(), Z.mean())
+    r2 = (X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2
+    v *= 1.0 + anomaly * np.exp(-r2 / anomaly_radius ** 2)
+    return np.clip(v, vmin, vmax)
+
+
+def layered_3d(n=256, dx=12.5, n_shots=8, nt=2000, order=8, precision="single", pml_cells=16, f0=6.0,
+               rec_patch=64, depth=None, seed=0):
+    """The 3D case: n^3 nodes, ``n_shots`` sources in a line at the surface,
+    a rec_patch x rec_patch receiver grid at the surface."""
+    v = layered_velocity_3d((n, n, n), dx, seed=seed)
+    width = pml_cells * dx
+    A = pml_amplitude(v.max(), width)
+    p = PML(width, A)
+    L = (n - 1) * dx
+    d = RectangularDomain((0.0, L, p, p), (0.0, L, p, p), (0.0, L, p, p))
+    m = CartesianMesh(d, n, n, n)
+    depth = 2 * dx if depth is None else depth
+    wav = RickerWavelet(f0)
+    xs = np.linspace(0.2 * L, 0.8 * L, n_shots)
+    rx = np.linspace(0.0, L, rec_patch)
+    shots = []
+    for x in xs:
+        recs = ReceiverSet(m, [PointReceiver(m, (a, b, depth)) for a in rx for b in rx])
+        shots.append(Shot(PointSource(m, (x, L / 2, depth), wav), recs))
+    dt = (1.0 / 6.0) * dx / v.max()
+    solver = ConstantDensityAcousticWave(m, spatial_accuracy_order=order, trange=trange_for_steps(dt, nt),
+                                         kernel_implementation="hip", precision=precision)
+    model = solver.ModelParameters(m, {"C": v.reshape(-1, 1)})
+    solver.model_parameters = model
+    return m, shots, solver, model

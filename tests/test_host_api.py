@@ -1,0 +1,177 @@
+"""Host-side pieces of the drop-in API (CPU tests): closed forms for the
+geometry, PML, point operators, wavelet and CFL step, the factory contract,
+model arithmetic, and the C ABI library's exports."""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pysit_amd as pa
+from tests.problems import c1_setup
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_pml_profile_closed_form():
+    # pysit/core/mesh.py:749-782 with domain.py:236-254: n = ceil(L/d);
+    # left[j] = A((n-j)/n)^2, right[j] = A((j+1)/n)^2
+    p = pa.PML(0.1, 100.0)
+    d = pa.RectangularDomain((0.0, 0.9, p, p), (0.0, 0.7, p, p))
+    m = pa.CartesianMesh(d, 91, 71)
+    n = m.x.lbc.n
+    assert n == 10 and m.z.rbc.n == 10
+    j = np.arange(n)
+    assert np.allclose(m.x.lbc.sigma, 100.0 * ((n - j) / n) ** 2, rtol=1e-15)
+    assert np.allclose(m.x.rbc.sigma, 100.0 * ((j + 1) / n) ** 2, rtol=1e-15)
+    assert m.shape(include_bc=True, as_grid=True) == [111, 91]
+
+
+def test_cfl_dt_and_nsteps_c1():
+    m, shots, solver, model = c1_setup(order=4)
+    solver.model_parameters = model
+    assert abs(solver.dt - 0.01 / (6 * 1.5)) < 1e-15
+    assert solver.nsteps == 1350
+    assert np.allclose(solver.ts()[:3], [0.0, solver.dt, 2 * solver.dt])
+
+
+def test_edge_padding_of_the_model():
+    m, shots, solver, model = c1_setup(order=4)
+    solver.model_parameters = model
+    Cp = solver.model_parameters.C.reshape(111, 91)
+    C = model.C.reshape(91, 71)
+    assert np.array_equal(Cp[10:101, 10:81], C)
+    assert np.array_equal(Cp[0, 10:81], C[0])
+    assert np.array_equal(Cp[:10, 0], np.full(10, C[0, 0]))
+
+
+def test_gaussian_point_operator():
+    m, shots, solver, model = c1_setup(order=4)
+    src = shots[0].sources
+    idx, w = src.entries()
+    assert idx.size == 49
+    # the weights integrate to one over the mesh, up to the 3-sigma truncation
+    assert abs(w.sum() * np.prod(m.deltas) - 1.0) < 2e-3
+    # closed form at the centre node: 1/(2 pi sigma^2) * exp(-|x-p|^2 / 2 sigma^2)
+    X, Z = m.mesh_coords()
+    px, pz = src.position
+    flat = np.ravel_multi_index((int(round(px / 0.01)), int(round(pz / 0.01))), (91, 71))
+    k = list(idx).index(flat)
+    want = 1.0 / (2 * np.pi * 0.01 ** 2) * np.exp(-((X.reshape(-1)[flat] - px) ** 2 + (Z.reshape(-1)[flat] - pz) ** 2)
+                                                  / (2 * 0.01 ** 2))
+    assert abs(w[k] - want) <= 1e-12 * want
+    # S = w * prod(deltas), S* = w
+    rec = shots[0].receivers
+    u = np.random.default_rng(0).standard_normal((m.dof(), 1))
+    d = rec.sample_data_from_array(u)
+    r0 = rec.receiver_list[0]
+    i0, w0 = r0.entries()
+    assert abs(d[0, 0] - np.sum(w0 * u[i0, 0]) * np.prod(m.deltas)) < 1e-12
+
+
+def test_delta_point_operator():
+    m, shots, solver, model = c1_setup(order=4)
+    s = pa.PointSource(m, (0.3, 0.2), pa.RickerWavelet(5.0), approximation="delta")
+    idx, w = s.entries()
+    assert idx.size == 1 and abs(w[0] - 1.0 / np.prod(m.deltas)) < 1e-9
+
+
+def test_ricker_closed_form():
+    w = pa.RickerWavelet(10.0)
+    sigma = 1.0 / (math.pi * 10.0 * math.sqrt(2.0))
+    t = np.linspace(0.0, 0.3, 301)
+    x = (t - 6 * sigma) / (math.sqrt(2.0) * sigma)
+    want = (1 - 2 * x ** 2) * np.exp(-x ** 2)
+    want[np.abs(want) < 1e-6] = 0.0
+    got = np.array([w(tt) for tt in t])
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-15)
+    assert abs(w(6 * sigma) - 1.0) < 1e-14
+
+
+def test_factory_dispatch():
+    m, shots, solver, model = c1_setup(order=4)
+    s = pa.ConstantDensityAcousticWave(m, spatial_accuracy_order=8, kernel_implementation="hip")
+    assert type(s).__name__ == "ConstantDensityAcousticTimeScalar_2D_hip"
+    assert s.supports["spatial_dimension"] == 2 and s.supports["temporal_integrator"] == "leap-frog"
+    s = pa.ConstantDensityAcousticWave(m)  # first partial match
+    assert type(s).__name__ == "ConstantDensityAcousticTimeScalar_2D_hip"
+    s = pa.ConstantDensityAcousticWave(m, spatial_accuracy_order=6, some_unknown_kwarg=3)
+    assert s.spatial_accuracy_order == 6
+    for bad in ({"spatial_accuracy_order": 10}, {"kernel_implementation": "cpp"},
+                {"equation_dynamics": "frequency"}):
+        with pytest.raises(pa.NoMatchError):
+            pa.ConstantDensityAcousticWave(m, **bad)
+    p = pa.PML(0.1, 10.0)
+    d3 = pa.RectangularDomain((0.0, 0.1, p, p), (0.0, 0.1, p, p), (0.0, 0.1, p, p))
+    m3 = pa.CartesianMesh(d3, 11, 11, 11)
+    s3 = pa.ConstantDensityAcousticWave(m3, spatial_accuracy_order=4)
+    assert type(s3).__name__ == "ConstantDensityAcousticTimeScalar_3D_hip"
+    assert s3.WavefieldVector.aux_names == ["psi", "Phix", "Phiy", "Phiz"]
+
+
+def test_model_arithmetic_is_linear_in_slowness_squared():
+    m, shots, solver, model = c1_setup(order=4)
+    p = model.perturbation(data=np.full((m.dof(), 1), 0.1))
+    m2 = model + p
+    assert np.allclose(m2.C, (model.C ** -2 + 0.1) ** -0.5)
+    diff = m2 - model
+    assert isinstance(diff, type(p)) and np.allclose(diff.data, 0.1)
+    assert abs(p.inner_product(p) - 0.01 * m.dof() * np.prod(m.deltas)) < 1e-12
+    assert model.validate()
+    bad = solver.ModelParameters(m, {"C": -model.C})
+    assert not bad.validate()
+
+
+def test_host_fold_is_transpose_of_edge_padding():
+    m, shots, solver, model = c1_setup(order=4)
+    rng = np.random.default_rng(0)
+    x = model.perturbation(data=rng.standard_normal((m.dof(), 1)))
+    y = rng.standard_normal((m.dof(include_bc=True), 1))
+    lhs = float(np.sum(x.with_padding(padding_mode="edge").data * y))
+    P = x.perturbation if hasattr(x, "perturbation") else None
+    yp = type(x)(m, padded=True, inputs=y)
+    rhs = float(np.sum(x.data * yp.add_padding().data))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_receiver_interpolation_zero_fill():
+    m, shots, solver, model = c1_setup(order=4)
+    rec = shots[0].receivers
+    ts = np.arange(11) * 0.1
+    rec.reset_time_series(ts)
+    rec.data = np.outer(2 * ts + 1, np.ones(rec.receiver_count))
+    tn = np.array([-0.05, 0.0, 0.333, 1.0, 1.2])
+    d = rec.interpolate_data(tn)
+    want = np.where((tn >= 0) & (tn <= 1.0), 2 * tn + 1, 0.0)
+    assert np.allclose(d[:, 3], want, atol=1e-14)
+
+
+def test_c_abi_exports_every_declared_symbol():
+    """libpysit_amd.so exports each entry point include/pysit_amd.h declares
+    (loaded, no device calls)."""
+    lib_path = os.path.join(ROOT, "pysit_amd", "libpysit_amd.so")
+    if not os.path.exists(lib_path):
+        pytest.skip("library not built (python -m pysit_amd._build)")
+    header = open(os.path.join(ROOT, "include", "pysit_amd.h")).read()
+    declared = set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(pysit_amd_\w+)\s*\(", header, re.M))
+    assert declared >= {"pysit_amd_batch_run", "pysit_amd_cda_step2d_f32", "pysit_amd_fold"}
+    L = ctypes.CDLL(lib_path)
+    for name in declared:
+        assert hasattr(L, name), name
+    from pysit_amd import _native
+    assert declared == set(_native.exported_symbols())
+    L.pysit_amd_abi_check.restype = ctypes.c_int
+    assert L.pysit_amd_abi_check(ctypes.sizeof(_native.Shot), ctypes.sizeof(_native.Batch)) == 0
+
+
+def test_solver_path_fails_loudly_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    m, shots, solver, model = c1_setup(order=4, tf=0.05)
+    tm = pa.TemporalModeling(solver)
+    with pytest.raises(Exception) as e:
+        tm.forward_model(shots[0], model, return_parameters=["simdata"])
+    assert "GPU" in str(e.value) or "ROCm" in str(e.value)
