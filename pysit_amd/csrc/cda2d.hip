@@ -366,28 +366,31 @@ __device__ __forceinline__ void sample_group(const pysit_amd_batch &B, int slot,
     if (live && g == 0) S->trace[(long long)k * S->nrec + rd.y] = acc;
 }
 
-template <typename T, int R, int MODE>
+// Grid: the first `rec_blocks` blocks sample receivers (dispatched first, so
+// their short gather chains overlap the tiles), the rest own tiles.  SINGLE
+// is the reference-contract step: one shot passed by value, every tile PML.
+template <typename T, int R, int MODE, bool SINGLE>
 __global__ __launch_bounds__(256) void k_step2d(const pysit_amd_batch B, const Step2DParams<T> P,
                                                 int slot, int seg_base, int n_tile_waves,
-                                                int n_rec_lanes)
+                                                int n_rec_lanes, int rec_blocks)
 {
     __shared__ T lds_f[4][Tile2D<T>::TX * 64];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int gw = xcd_block(blockIdx.x, gridDim.x) * 4 + w;
-    if (gw < n_tile_waves) {
-        if (P.single) {
-            tile2d<T, R, MODE>(B, P, P.shot0, gw, 1, gw, slot, seg_base, lane, lds_f[w]);
-        } else {
-            const int4 td = reinterpret_cast<const int4 *>(B.tiles)[gw];
-            tile2d<T, R, MODE>(B, P, B.shots[td.x], gw, td.z, td.y, slot, seg_base, lane, lds_f[w]);
-        }
+    const int b = blockIdx.x;
+    if (MODE != 2 && b < rec_blocks) {
+        const int rw = b * 4 + w;
+        const int rec = rw * 4 + (lane >> 4);
+        if (rw * 4 < n_rec_lanes) sample_group<T>(B, slot, rec < n_rec_lanes ? rec : B.n_rec, lane & 15);
         return;
     }
-    if (MODE != 2) {
-        const int rec = (gw - n_tile_waves) * 4 + (lane >> 4);
-        if (__builtin_amdgcn_readfirstlane((gw - n_tile_waves) * 4) < n_rec_lanes)
-            sample_group<T>(B, slot, rec < n_rec_lanes ? rec : B.n_rec, lane & 15);
+    const int gw = xcd_block(b - rec_blocks, gridDim.x - rec_blocks) * 4 + w;
+    if (gw >= n_tile_waves) return;
+    if (SINGLE) {
+        tile2d<T, R, MODE>(B, P, P.shot0, gw, 1, gw, slot, seg_base, lane, lds_f[w]);
+    } else {
+        const int4 td = reinterpret_cast<const int4 *>(B.tiles)[gw];
+        tile2d<T, R, MODE>(B, P, B.shots[td.x], gw, td.z, td.y, slot, seg_base, lane, lds_f[w]);
     }
 }
 
@@ -395,12 +398,17 @@ template <typename T, int R, int MODE>
 static int launch2d(const pysit_amd_batch &B, const Step2DParams<T> &P, int slot, int seg_base,
                     int n_tiles, int n_recs, hipStream_t st)
 {
-    const int rec_waves = (MODE != 2) ? (n_recs + 3) / 4 : 0;
-    const int waves = n_tiles + rec_waves;
-    if (waves == 0) return PYSIT_AMD_OK;
-    const int blocks = (waves + 3) / 4;
-    hipLaunchKernelGGL((k_step2d<T, R, MODE>), dim3(blocks), dim3(256), 0, st, B, P, slot, seg_base,
-                       n_tiles, (MODE != 2) ? n_recs : 0);
+    const int recs = (MODE != 2) ? n_recs : 0;
+    // 16 receivers per block, block count rounded to whole XCD rounds
+    const int rec_blocks = recs ? ((recs + 15) / 16 + 7) / 8 * 8 : 0;
+    const int tile_blocks = (n_tiles + 3) / 4;
+    if (rec_blocks + tile_blocks == 0) return PYSIT_AMD_OK;
+    if (P.single)
+        hipLaunchKernelGGL((k_step2d<T, R, MODE, true>), dim3(rec_blocks + tile_blocks), dim3(256), 0, st,
+                           B, P, slot, seg_base, n_tiles, recs, rec_blocks);
+    else
+        hipLaunchKernelGGL((k_step2d<T, R, MODE, false>), dim3(rec_blocks + tile_blocks), dim3(256), 0, st,
+                           B, P, slot, seg_base, n_tiles, recs, rec_blocks);
     PA_HIP(hipGetLastError());
     return PYSIT_AMD_OK;
 }
