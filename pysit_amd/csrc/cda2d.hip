@@ -8,16 +8,29 @@
 //
 // Mapping to CDNA: one wavefront owns a tile of 64 consecutive z (the
 // contiguous axis: every row access is one coalesced 256 B transaction) by
-// TX rows along x (16 in FP32, 8 in FP64).  A tile is processed in register
-// blocks of NB rows: all loads of a block (the NB + 2r rows of u its x
-// derivatives need, u_{k-1}, the imaging accumulator) are issued up front so
-// each wave keeps tens of loads in flight, then the NB rows are computed
-// fully unrolled.  z derivatives take their neighbours from other lanes with
-// ds_bpermute (no LDS storage, no fences, so the compiler is free to hoist
-// loads); lanes 0..r-1 and 64-r..63 also carry the r-node halo beyond each
-// side of the tile.  Tiles farther than r nodes from every PML node take a
-// path without Phi; in PML tiles Phi is read and written only where it can
-// be nonzero (PML nodes), except for the reference-contract step entry point,
+// TX rows along x (16 in FP32, 8 in FP64).  The z tiles of a shot start at
+// 0, 64, 128, ... except the last, which ends on the last node; the node
+// ranges the tiles own are disjoint (a tile lane that another tile owns only
+// loads), so the PML band at the far z side costs one PML tile, not two.
+// A tile is processed in register blocks of NB rows: all loads of a block
+// (the NB + 2r rows of u its x derivatives need, u_{k-1}, the model, the
+// imaging accumulator) are issued up front, then the rows are computed
+// fully unrolled.  z derivatives take their neighbours from other lanes through
+// an LDS row per wavefront; lanes 0..r-1 and 64-r..63 also carry the r-node
+// halo beyond each side of the tile.
+//
+// Addressing: every array is reached through a raw buffer resource with a
+// per-lane byte offset fixed for the whole tile (z, or an out-of-range value
+// for masked lanes) and a wave-uniform row offset in an SGPR, so the row loop
+// spends no vector instructions on addresses or masks.  In a batch the six
+// state buffers (u, Phix, Phiz, two time levels each) lie in one region and
+// share one resource.
+//
+// Two row paths: tiles whose nodes and r-neighbourhoods contain no PML node,
+// no edge node and nothing outside the grid (the host flags them) run the
+// plain leapfrog update with no Phi traffic and no checks; the other tiles
+// run the full PML update, reading and writing Phi only where it can be
+// nonzero (PML nodes) except for the reference-contract step entry point,
 // which treats Phi as arbitrary input.
 //
 // Fused per launch: source / adjoint-source injection (per-tile entry
@@ -44,8 +57,27 @@ template <> struct Tile2D<double> { static constexpr int TX = 8; };
 template <typename T, int MODE, bool FULL>
 constexpr int block_rows()
 {
-    return Tile2D<T>::TX / ((FULL ? 2 : 1) * (MODE == 2 ? 2 : 1));
+    return Tile2D<T>::TX / ((FULL ? 2 : 1) * (MODE >= 2 ? 2 : 1));
 }
+
+// state buffers of one step: u_k, u_{k-1} (overwritten by u_{k+1}),
+// Phix_k, Phix_{k+1}, Phiz_k, Phiz_{k+1}
+enum { F_U = 0, F_UO, F_PX, F_PXO, F_PZ, F_PZO, NF2 };
+
+template <bool PACKED> struct State2D;
+// batch: one resource over the state region plus a byte offset per buffer
+template <> struct State2D<true> {
+    Rsrc r;
+    int o[NF2];
+    __device__ __forceinline__ Rsrc rs(int) const { return r; }
+    __device__ __forceinline__ int so(int f) const { return o[f]; }
+};
+// reference-contract step: caller-owned arrays, one resource each
+template <> struct State2D<false> {
+    Rsrc r[NF2];
+    __device__ __forceinline__ Rsrc rs(int f) const { return r[f]; }
+    __device__ __forceinline__ int so(int) const { return 0; }
+};
 
 template <typename T>
 struct Step2DParams {
@@ -54,108 +86,167 @@ struct Step2DParams {
     const T *rhs;          // dense rhs (step entry point only), may be null
     int single;            // 1: one shot given by value, tiles by arithmetic
     int general_phi;       // 1: Phi may be nonzero anywhere (step entry point)
+    const void *state;     // batch: base of the state region
+    int state_bytes;       //   its extent
+    int fo[3][2];          //   byte offsets of u, Phix, Phiz buffers 0 and 1
     pysit_amd_shot shot0;  // used when single
 };
 
-// value of lane `src_byte/4` of v (32- or 64-bit)
-__device__ __forceinline__ float lane_read(float v, int src_byte)
-{
-    return __int_as_float(__builtin_amdgcn_ds_bpermute(src_byte, __float_as_int(v)));
-}
-__device__ __forceinline__ double lane_read(double v, int src_byte)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_ds_bpermute(src_byte, (int)(b & 0xffffffffLL));
-    const int hi = __builtin_amdgcn_ds_bpermute(src_byte, (int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-// Per-tile constants shared by the row blocks.  Every field access goes
-// through a raw buffer resource based at this shot's region: masked lanes use
-// an out-of-range offset, so loads and stores need no branches and the whole
-// register block is one straight-line region the scheduler can reorder.
-template <typename T, int R>
+// Per-tile constants shared by the row blocks.
+template <typename T, int R, bool PACKED>
 struct Ctx2D {
-    Rsrc u, uo, px, pxo, pz, pzo, model, m1, dwi, dwo, fst, ic, rhs;
-    bool has_fst, has_dwo, has_rhs, has_inj;
+    State2D<PACKED> s;
+    Rsrc model, m1, dw, fst, ic, rhs;  // dw: read (adjoint, Born) or written (forward)
+    bool has_fst, has_dwo, has_rhs, has_inj, img, general_phi;
     const T *finj;  // LDS row sources of this wave
     const T *xl, *xr;
-    int nxl, nxr, nzl, nzr;
-    int nx, nz, z, zc, hz, hzc, lane, mcol, mx_off, mx_lo, mx_hi, mnz;
-    bool zin, hin, general_phi, img;
+    int nxl, nxr;
+    int nx, nz, lane;
+    int rowb;                         // bytes per grid row
+    int shb, shb8;                    // shot offset in slot-major stores (T) and in the accumulator (FP64)
+    int mx_off, mx_lo, mx_hi, mrowb;  // model row window, bytes per model row
+    // per-lane byte offsets, kOOB where masked
+    int zv;    // centre node inside the grid
+    int hv;    // halo node inside the grid
+    int ov;    // owned node: u_{k-1} and every store
+    int ov8;   // owned node, FP64 arrays
+    int mv;    // model column (edge padding by clamping)
+    int zpv;   // centre node where Phi can be nonzero without a PML row
+    int hpv;   // halo node likewise
+    bool zedge;
     T szk, szm;  // lane z profile (kernel, mesh)
-    int srcp[R + 1], srcm[R + 1];
-    bool inp[R + 1], inm[R + 1];
+    T *zbuf;     // this wave's z-exchange rows (LDS), 4 x ZROW
+    int hslot;   // where this lane's halo value goes in a row
 };
 
-template <typename T, int R, int NB, int MODE, bool FULL>
-__device__ __forceinline__ void rows2d(const Ctx2D<T, R> &c, const Step2DParams<T> &P, int xb, int jb)
+// z-exchange row: slots 0..R-1 left halo, R..R+63 the lanes, R+64..2R+63
+// right halo, then a parking slot per lane for lanes that hold no halo
+template <int R> constexpr int zrow_len() { return 2 * R + 128; }
+
+// z neighbours at distance 1..R of every lane: each lane writes its value
+// (and halo lanes their halo value) into an LDS row, then reads the 2R
+// neighbours at fixed offsets.  LDS operations of one wavefront execute in
+// order; wave_sync keeps the compiler from moving the reads above the writes,
+// and rows alternate between buffers so the next row's writes cannot move
+// above this row's reads either.
+template <typename T, int R, bool PACKED>
+__device__ __forceinline__ void z_taps(const Ctx2D<T, R, PACKED> &c, int buf, T v, T h, T (&zp)[R + 1],
+                                       T (&zm)[R + 1])
 {
-    const int nz = c.nz, nx = c.nx, z = c.z;
-    // ---- loads of the block -------------------------------------------------
+    T *row = c.zbuf + buf * zrow_len<R>();
+    row[c.lane + R] = v;
+    row[c.hslot] = h;
+    wave_sync();
+#pragma unroll
+    for (int i = 1; i <= R; ++i) {
+        zp[i] = row[c.lane + R + i];
+        zm[i] = row[c.lane + R - i];
+    }
+}
+
+// Interior rows: no PML node within r, no edge node, every row in the grid.
+template <typename T, int R, int NB, int MODE, bool PACKED>
+__device__ __forceinline__ void rows_int(const Ctx2D<T, R, PACKED> &c, const Step2DParams<T> &P, int xb, int jb)
+{
+    const int rb = c.rowb;
     T uc[NB + 2 * R];
 #pragma unroll
-    for (int j = 0; j < NB + 2 * R; ++j) {
-        const int xx = xb - R + j;
-        uc[j] = buf_ld<T>(c.u, boff<T>(c.zin && xx >= 0 && xx < nx, (long long)xx * nz + z));
-    }
+    for (int j = 0; j < NB + 2 * R; ++j) uc[j] = buf_ld<T>(c.s.rs(F_U), c.zv, c.s.so(F_U) + (xb - R + j) * rb);
     T uh[NB], um[NB], cv[NB], fv[NB], dv[NB], mv[NB];
     double icv[NB];
+    const bool rd = MODE == 3 || (MODE == 2 && c.img);
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
         const int x = xb + j;
-        const long long row = (long long)x * nz;
-        const bool st = c.zin && x < nx;
-        uh[j] = buf_ld<T>(c.u, boff<T>(c.hin && x < nx, row + c.hz));
-        um[j] = buf_ld<T>(c.uo, boff<T>(st, row + z));
-        const long long mrow = (long long)clampi(x - c.mx_off, c.mx_lo, c.mx_hi) * c.mnz;
-        cv[j] = buf_ld<T>(c.model, boff<T>(true, mrow + c.mcol));
-        fv[j] = c.has_rhs ? buf_ld<T>(c.rhs, boff<T>(st, row + z)) : T(0);
-        if (c.has_inj) fv[j] += c.finj[(jb + j) * 64 + c.lane];
-        dv[j] = (MODE == 3 || (MODE == 2 && c.img)) ? buf_ld<T>(c.dwi, boff<T>(st, row + z)) : T(0);
-        mv[j] = (MODE == 3) ? buf_ld<T>(c.m1, boff<T>(true, mrow + c.mcol)) : T(0);
-        icv[j] = (MODE == 2 && c.img) ? buf_ld<double>(c.ic, boff<double>(st, row + z)) : 0.0;
+        const int xr = x * rb;
+        uh[j] = buf_ld<T>(c.s.rs(F_U), c.hv, c.s.so(F_U) + xr);
+        um[j] = buf_ld<T>(c.s.rs(F_UO), c.ov, c.s.so(F_UO) + xr);
+        const int mr = clampi(x - c.mx_off, c.mx_lo, c.mx_hi) * c.mrowb;
+        cv[j] = buf_ld<T>(c.model, c.mv, mr);
+        fv[j] = c.has_inj ? c.finj[(jb + j) * 64 + c.lane] : T(0);
+        dv[j] = rd ? buf_ld<T>(c.dw, c.ov, c.shb + xr) : T(0);
+        mv[j] = (MODE == 3) ? buf_ld<T>(c.m1, c.mv, mr) : T(0);
+        icv[j] = (MODE == 2 && c.img) ? buf_ld<double>(c.ic, c.ov8, c.shb8 + x * c.nz * 8) : 0.0;
     }
-    // PML tiles: Phix along x for the block, Phiz centre and halo per row,
-    // read only where Phi can be nonzero (PML nodes)
-    T pc[FULL ? NB + 2 * R : 1], pv[FULL ? NB : 1], ph[FULL ? NB : 1];
-    if (FULL) {
-        const bool zp = c.general_phi || c.z < c.nzl || c.z >= nz - c.nzr;
-        const bool hzp = c.general_phi || c.hz < c.nzl || c.hz >= nz - c.nzr;
-#pragma unroll
-        for (int j = 0; j < NB + 2 * R; ++j) {
-            const int xx = xb - R + j;
-            const bool xp = xx < c.nxl || xx >= nx - c.nxr;
-            pc[j] = buf_ld<T>(c.px, boff<T>((xp || zp) && c.zin && xx >= 0 && xx < nx, (long long)xx * nz + z));
-        }
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            const int x = xb + j;
-            const long long row = (long long)x * nz;
-            const bool xp = x < c.nxl || x >= nx - c.nxr;
-            pv[j] = buf_ld<T>(c.pz, boff<T>((xp || zp) && c.zin && x < nx, row + z));
-            ph[j] = buf_ld<T>(c.pz, boff<T>((xp || hzp) && c.hin && x < nx, row + c.hz));
-        }
-    }
-    // ---- rows ------------------------------------------------------------------
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
         const int x = xb + j;
-        const bool st = c.zin && x < nx;
-        const long long row = (long long)x * nz;
+        const int xr = x * rb;
         const T v = uc[R + j];
         T f = fv[j];
         if (MODE == 3) f += mv[j] * (-dv[j]);  // Born source: m1_pad * (-dWaveOp0_k)
-
-        // z neighbours from other lanes (halo lanes supply the tile edges)
         T zp[R + 1], zm[R + 1];
+        z_taps<T, R, PACKED>(c, 2 * (j & 1), v, uh[j], zp, zm);
+        T lx = T(0), lz = T(0);
 #pragma unroll
-        for (int i = 1; i <= R; ++i) {
-            const T a = lane_read(v, c.srcp[i]), b = lane_read(uh[j], c.srcp[i]);
-            const T d = lane_read(v, c.srcm[i]), e = lane_read(uh[j], c.srcm[i]);
-            zp[i] = c.inp[i] ? a : b;
-            zm[i] = c.inm[i] ? d : e;
+        for (int i = R; i >= 1; --i) {
+            lx = fma(P.cx.d2[i], uc[R + j + i] + uc[R + j - i], lx);
+            lz = fma(P.cz.d2[i], zp[i] + zm[i], lz);
         }
+        lx = fma(P.cx.d2[0], v, lx);
+        lz = fma(P.cz.d2[0], v, lz);
+        const T lap = lx + lz;
+        const T c2 = cv[j] * cv[j];
+        const T src = f + lap;
+        const T up = fma(P.tc.dt2, c2 * src, T(2) * v - um[j]);
+        if (MODE == 2 && c.img) buf_st(c.ic, c.ov8, c.shb8 + x * c.nz * 8, icv[j] + (double)v * (double)dv[j]);
+        buf_st(c.s.rs(F_UO), c.ov, c.s.so(F_UO) + xr, up);
+        if (MODE == 1 && c.has_dwo) buf_st(c.dw, c.ov, c.shb + xr, c2 * src);
+        if (c.has_fst) buf_st(c.fst, c.ov, c.shb + xr, v);
+    }
+}
+
+// Rows with the PML update, edges and rows beyond the grid.
+template <typename T, int R, int NB, int MODE, bool PACKED>
+__device__ __forceinline__ void rows_full(const Ctx2D<T, R, PACKED> &c, const Step2DParams<T> &P, int xb, int jb)
+{
+    const int rb = c.rowb, nx = c.nx;
+    T uc[NB + 2 * R], pc[NB + 2 * R];
+#pragma unroll
+    for (int j = 0; j < NB + 2 * R; ++j) {
+        const int xx = xb - R + j;
+        const bool rin = xx >= 0 && xx < nx;
+        const bool xp = xx < c.nxl || xx >= nx - c.nxr;
+        const int xr = (rin ? xx : 0) * rb;
+        uc[j] = buf_ld<T>(c.s.rs(F_U), rin ? c.zv : kOOB, c.s.so(F_U) + xr);
+        pc[j] = buf_ld<T>(c.s.rs(F_PX), rin ? (xp ? c.zv : c.zpv) : kOOB, c.s.so(F_PX) + xr);
+    }
+    T uh[NB], um[NB], cv[NB], fv[NB], dv[NB], mv[NB], pv[NB], ph[NB];
+    double icv[NB];
+    const bool rd = MODE == 3 || (MODE == 2 && c.img);
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const int x = xb + j;
+        const bool own = x < nx;
+        const int xs = own ? x : nx - 1;
+        const int xr = xs * rb;
+        const bool xp = x < c.nxl || x >= nx - c.nxr;
+        const int ovr = own ? c.ov : kOOB;
+        uh[j] = buf_ld<T>(c.s.rs(F_U), own ? c.hv : kOOB, c.s.so(F_U) + xr);
+        um[j] = buf_ld<T>(c.s.rs(F_UO), ovr, c.s.so(F_UO) + xr);
+        const int mr = clampi(xs - c.mx_off, c.mx_lo, c.mx_hi) * c.mrowb;
+        cv[j] = buf_ld<T>(c.model, c.mv, mr);
+        T f = c.has_rhs ? buf_ld<T>(c.rhs, ovr, xr) : T(0);
+        if (c.has_inj) f += c.finj[(jb + j) * 64 + c.lane];
+        fv[j] = f;
+        dv[j] = rd ? buf_ld<T>(c.dw, ovr, c.shb + xr) : T(0);
+        mv[j] = (MODE == 3) ? buf_ld<T>(c.m1, c.mv, mr) : T(0);
+        icv[j] = (MODE == 2 && c.img) ? buf_ld<double>(c.ic, own ? c.ov8 : kOOB, c.shb8 + xs * c.nz * 8) : 0.0;
+        pv[j] = buf_ld<T>(c.s.rs(F_PZ), own ? (xp ? c.zv : c.zpv) : kOOB, c.s.so(F_PZ) + xr);
+        ph[j] = buf_ld<T>(c.s.rs(F_PZ), own ? (xp ? c.hv : c.hpv) : kOOB, c.s.so(F_PZ) + xr);
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const int x = xb + j;
+        const bool own = x < nx;
+        const int xr = (own ? x : nx - 1) * rb;
+        const int ovr = own ? c.ov : kOOB;
+        const T v = uc[R + j];
+        T f = fv[j];
+        if (MODE == 3) f += mv[j] * (-dv[j]);
+        T zp[R + 1], zm[R + 1], pzp[R + 1], pzm[R + 1];
+        z_taps<T, R, PACKED>(c, 2 * (j & 1), v, uh[j], zp, zm);
+        z_taps<T, R, PACKED>(c, 2 * (j & 1) + 1, pv[j], ph[j], pzp, pzm);
         T lx = T(0), lz = T(0);
 #pragma unroll
         for (int i = R; i >= 1; --i) {
@@ -167,149 +258,149 @@ __device__ __forceinline__ void rows2d(const Ctx2D<T, R> &c, const Step2DParams<
         const T lap = lx + lz;
         const T c2 = cv[j] * cv[j];
         const T u_m = um[j];
-        const bool edge = (x == 0) | (x == nx - 1) | (z == 0) | (z >= nz - 1);
+        const bool edge = (x == 0) | (x == nx - 1) | c.zedge;
 
-        T up, dw, pxn = T(0), pzn = T(0);
-        bool phi_store = false;
-        if (!FULL) {
-            const T src = f + lap;
-            up = fma(P.tc.dt2, c2 * src, T(2) * v - u_m);
-            dw = c2 * src;
-        } else {
-            T pzp[R + 1], pzm[R + 1];
+        T dux = T(0), duz = T(0), dpx = T(0), dpz = T(0);
 #pragma unroll
-            for (int i = 1; i <= R; ++i) {
-                const T a = lane_read(pv[j], c.srcp[i]), b = lane_read(ph[j], c.srcp[i]);
-                const T d = lane_read(pv[j], c.srcm[i]), e = lane_read(ph[j], c.srcm[i]);
-                pzp[i] = c.inp[i] ? a : b;
-                pzm[i] = c.inm[i] ? d : e;
-            }
-            T dux = T(0), duz = T(0), dpx = T(0), dpz = T(0);
-#pragma unroll
-            for (int i = R; i >= 1; --i) {
-                dux = fma(P.cx.d1[i - 1], uc[R + j + i] - uc[R + j - i], dux);
-                duz = fma(P.cz.d1[i - 1], zp[i] - zm[i], duz);
-                dpx = fma(P.cx.d1[i - 1], pc[R + j + i] - pc[R + j - i], dpx);
-                dpz = fma(P.cz.d1[i - 1], pzp[i] - pzm[i], dpz);
-            }
-            const T src = ((f + lap) + dpx) + dpz;
-            const T sx = sigma_kernel<T>(c.xl, c.nxl, c.xr, c.nxr, min(x, nx - 2), nx);
-            const T sxm = sigma_mesh<T>(c.xl, c.nxl, c.xr, c.nxr, min(x, nx - 1), nx);
-            const T szk = c.szk, szm = c.szm;
-            const T dt = P.tc.dt;
-            const T pxc = pc[R + j], pzc = pv[j];
-            const bool pml = sx != T(0) || szk != T(0);
-            // damped update (PML node) and plain update, selected per lane
-            const T pxd = (pxc - dt * sx * pxc) + dt * (szk - sx) * dux;
-            const T pzd = (pzc - dt * szk * pzc) + dt * (sx - szk) * duz;
-            const T ss = sx + szk;
-            const T fac1 = P.tc.two_dt2 / (T(2) + dt * ss);
-            const T fac2 = c2 * src - (u_m - T(2) * v) * P.tc.inv_dt2 + ss * u_m * P.tc.inv_2dt -
-                           (sx * szk) * v;
-            const T upd = fac1 * fac2;
-            const T upi = fma(P.tc.dt2, c2 * src, T(2) * v - u_m);
-            pxn = pml ? pxd : pxc;
-            pzn = pml ? pzd : pzc;
-            up = pml ? upd : upi;
-            phi_store = pml || c.general_phi;
-            // dWaveOp with the mesh profile: the step identity gives
-            // (u+ - 2u + u-)/dt^2 = c^2 src - sum(sk)(u+ - u-)/(2dt) - prod(sk) u
-            dw = c2 * src + ((sxm + szm) - (sx + szk)) * (up - u_m) * P.tc.inv_2dt +
-                 (sxm * szm - sx * szk) * v;
+        for (int i = R; i >= 1; --i) {
+            dux = fma(P.cx.d1[i - 1], uc[R + j + i] - uc[R + j - i], dux);
+            duz = fma(P.cz.d1[i - 1], zp[i] - zm[i], duz);
+            dpx = fma(P.cx.d1[i - 1], pc[R + j + i] - pc[R + j - i], dpx);
+            dpz = fma(P.cz.d1[i - 1], pzp[i] - pzm[i], dpz);
         }
+        const T src = ((f + lap) + dpx) + dpz;
+        const int xs = own ? x : nx - 1;
+        const T sx = sigma_kernel<T>(c.xl, c.nxl, c.xr, c.nxr, max(0, min(xs, nx - 2)), nx);
+        const T sxmj = sigma_mesh<T>(c.xl, c.nxl, c.xr, c.nxr, xs, nx);
+        const T szk = c.szk, szm = c.szm;
+        const T dt = P.tc.dt;
+        const T pxc = pc[R + j], pzc = pv[j];
+        const bool pml = sx != T(0) || szk != T(0);
+        // damped update (PML node) and plain update, selected per lane
+        const T pxd = (pxc - dt * sx * pxc) + dt * (szk - sx) * dux;
+        const T pzd = (pzc - dt * szk * pzc) + dt * (sx - szk) * duz;
+        const T ss = sx + szk;
+        const T fac1 = P.tc.two_dt2 / (T(2) + dt * ss);
+        const T fac2 = c2 * src - (u_m - T(2) * v) * P.tc.inv_dt2 + ss * u_m * P.tc.inv_2dt - (sx * szk) * v;
+        const T upd = fac1 * fac2;
+        const T upi = fma(P.tc.dt2, c2 * src, T(2) * v - u_m);
+        T pxn = pml ? pxd : pxc;
+        T pzn = pml ? pzd : pzc;
+        T up = pml ? upd : upi;
+        bool phi_store = pml || c.general_phi;
+        // dWaveOp with the mesh profile: the step identity gives
+        // (u+ - 2u + u-)/dt^2 = c^2 src - sum(sk)(u+ - u-)/(2dt) - prod(sk) u
+        T dw = c2 * src + ((sxmj + szm) - (sx + szk)) * (up - u_m) * P.tc.inv_2dt + (sxmj * szm - sx * szk) * v;
         if (edge) {
             up = T(0);
             pxn = T(0);
             pzn = T(0);
             dw = T(0);
-            phi_store = FULL;
+            phi_store = true;
         }
-        const int o = boff<T>(st, row + z);
-        if (MODE == 2 && c.img) buf_st(c.ic, boff<double>(st, row + z), icv[j] + (double)v * (double)dv[j]);
-        buf_st(c.uo, o, up);
-        if (FULL) {
-            const int op = (st && phi_store) ? o : kOOB;
-            buf_st(c.pxo, op, pxn);
-            buf_st(c.pzo, op, pzn);
-        }
-        if (MODE == 1 && c.has_dwo) buf_st(c.dwo, o, dw);
-        if (c.has_fst) buf_st(c.fst, o, v);
+        if (MODE == 2 && c.img)
+            buf_st(c.ic, own ? c.ov8 : kOOB, c.shb8 + (own ? x : nx - 1) * c.nz * 8, icv[j] + (double)v * (double)dv[j]);
+        buf_st(c.s.rs(F_UO), ovr, c.s.so(F_UO) + xr, up);
+        const int op = phi_store ? ovr : kOOB;
+        buf_st(c.s.rs(F_PXO), op, c.s.so(F_PXO) + xr, pxn);
+        buf_st(c.s.rs(F_PZO), op, c.s.so(F_PZO) + xr, pzn);
+        if (MODE == 1 && c.has_dwo) buf_st(c.dw, ovr, c.shb + xr, dw);
+        if (c.has_fst) buf_st(c.fst, ovr, c.shb + xr, v);
     }
 }
 
-template <typename T, int R, int MODE>
+template <typename T, int R, int MODE, bool SINGLE>
 __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DParams<T> &P,
-                                       const pysit_amd_shot &S, int tile, int flags, int tloc,
-                                       int slot, int seg_base, int lane, T *lds_f)
+                                       const pysit_amd_shot &S, int tile, int full, int tloc,
+                                       int slot, int seg_base, int lane, T *lds_f, T *zbuf)
 {
+    constexpr bool PACKED = !SINGLE;
     constexpr int TX = Tile2D<T>::TX;
+    constexpr int SZ = sizeof(T);
     const int k = slot - S.kshift;
     if (k < 0 || k >= S.nt) return;
-    Ctx2D<T, R> c;
-    c.nx = S.nx;
-    c.nz = S.nz;
-    const int tz = tloc % S.ntz, tx = tloc / S.ntz;
-    const int z0 = tz * 64;
-    c.lane = lane;
-    c.z = z0 + lane;
-    c.zin = c.z < c.nz;
+    Ctx2D<T, R, PACKED> c;
+    const int nx = S.nx, nz = S.nz;
+    c.nx = nx;
+    c.nz = nz;
+    // z tiles: 0, 64, ... and a last one ending on node nz-1; owned ranges disjoint
+    const int ntz = S.ntz;
+    const int tz = tloc % ntz, tx = tloc / ntz;
+    const int zlast = nz > 64 ? nz - 64 : 0;
+    const int z0 = (tz == ntz - 1) ? zlast : tz * 64;
+    const int own_hi = (tz == ntz - 1) ? nz : min(z0 + 64, zlast);
     const int x0 = tx * TX;
-    const bool full = flags & 1;
+    c.lane = lane;
+    const int z = z0 + lane;
+    const bool zin = z < nz;
+    const bool own = z < own_hi;
+    c.zv = zin ? z * SZ : kOOB;
+    c.ov = own ? z * SZ : kOOB;
+    c.ov8 = own ? z * 8 : kOOB;
+    // halo lanes: 0..R-1 hold z0+64+lane, 64-R..63 hold z0+lane-64
+    const int hz = lane < R ? z0 + 64 + lane : (lane >= 64 - R ? z0 + lane - 64 : -1);
+    const bool hin = hz >= 0 && hz < nz;
+    c.hv = hin ? hz * SZ : kOOB;
+    const bool zp = P.general_phi || z < S.nzl || z >= nz - S.nzr;
+    const bool hzp = P.general_phi || hz < S.nzl || hz >= nz - S.nzr;
+    c.zpv = zp ? c.zv : kOOB;
+    c.hpv = hzp ? c.hv : kOOB;
+    c.zedge = z == 0 || z >= nz - 1;
     // which buffer holds time level k: parity of the steps this propagation took
     const int taken = (MODE == 2) ? (S.nt - 1 - k) : k;
     const int cur = taken & 1;
-    const long long off = S.off;
-    const long long nbytes = (long long)c.nx * c.nz * (long long)sizeof(T);
-    c.u = make_rsrc((const T *)B.u[cur] + off, nbytes);
-    c.uo = make_rsrc((T *)B.u[cur ^ 1] + off, nbytes);
-    c.px = make_rsrc((const T *)B.phix[cur] + off, nbytes);
-    c.pxo = make_rsrc((T *)B.phix[cur ^ 1] + off, nbytes);
-    c.pz = make_rsrc((const T *)B.phiz[cur] + off, nbytes);
-    c.pzo = make_rsrc((T *)B.phiz[cur ^ 1] + off, nbytes);
-    c.model = make_rsrc(S.model, (long long)(S.mx_hi + 1) * S.mnz * (long long)sizeof(T));
-    c.mcol = clampi(c.z - S.mz_off, S.mz_lo, S.mz_hi);
+    c.rowb = nz * SZ;
+    c.shb = (int)(S.off * SZ);
+    c.shb8 = (int)(S.off * 8);
+    if constexpr (PACKED) {
+        c.s.r = make_rsrc(P.state, P.state_bytes);
+        c.s.o[F_U] = P.fo[0][cur] + c.shb;
+        c.s.o[F_UO] = P.fo[0][cur ^ 1] + c.shb;
+        c.s.o[F_PX] = P.fo[1][cur] + c.shb;
+        c.s.o[F_PXO] = P.fo[1][cur ^ 1] + c.shb;
+        c.s.o[F_PZ] = P.fo[2][cur] + c.shb;
+        c.s.o[F_PZO] = P.fo[2][cur ^ 1] + c.shb;
+    } else {
+        const long long nb = (long long)nx * nz * SZ;
+        c.s.r[F_U] = make_rsrc(B.u[cur], nb);
+        c.s.r[F_UO] = make_rsrc(B.u[cur ^ 1], nb);
+        c.s.r[F_PX] = make_rsrc(B.phix[cur], nb);
+        c.s.r[F_PXO] = make_rsrc(B.phix[cur ^ 1], nb);
+        c.s.r[F_PZ] = make_rsrc(B.phiz[cur], nb);
+        c.s.r[F_PZO] = make_rsrc(B.phiz[cur ^ 1], nb);
+    }
+    c.model = make_rsrc(S.model, (long long)(S.mx_hi + 1) * S.mnz * SZ);
+    c.mv = clampi(z - S.mz_off, S.mz_lo, S.mz_hi) * SZ;
     c.mx_off = S.mx_off;
     c.mx_lo = S.mx_lo;
     c.mx_hi = S.mx_hi;
-    c.mnz = S.mnz;
+    c.mrowb = S.mnz * SZ;
     c.xl = (const T *)S.xl;
     c.xr = (const T *)S.xr;
     c.nxl = S.nxl;
     c.nxr = S.nxr;
-    c.nzl = S.nzl;
-    c.nzr = S.nzr;
     c.general_phi = P.general_phi;
     c.has_rhs = P.rhs != nullptr;
-    c.rhs = make_rsrc(P.rhs, c.has_rhs ? nbytes : 0);
-    // halo lanes: 0..R-1 hold z0+64+lane, 64-R..63 hold z0+lane-64
-    c.hz = lane < R ? z0 + 64 + lane : (lane >= 64 - R ? z0 + lane - 64 : -1);
-    c.hin = c.hz >= 0 && c.hz < c.nz;
-#pragma unroll
-    for (int i = 1; i <= R; ++i) {
-        c.srcp[i] = ((lane + i) & 63) << 2;
-        c.srcm[i] = ((lane - i) & 63) << 2;
-        c.inp[i] = lane + i <= 63;
-        c.inm[i] = lane - i >= 0;
-    }
+    c.rhs = make_rsrc(P.rhs, c.has_rhs ? (long long)nx * nz * SZ : 0);
+    c.zbuf = zbuf;
+    c.hslot = lane < R ? R + 64 + lane : (lane >= 64 - R ? lane - (64 - R) : 2 * R + 64 + lane);
     const T *zl = (const T *)S.zl, *zr = (const T *)S.zr;
     c.szk = c.szm = T(0);
-    if (full && c.zin) {
-        c.szk = (c.z <= c.nz - 2) ? sigma_kernel<T>(zl, S.nzl, zr, S.nzr, c.z, c.nz) : T(0);
-        c.szm = sigma_mesh<T>(zl, S.nzl, zr, S.nzr, c.z, c.nz);
+    if (full && zin) {
+        c.szk = (z <= nz - 2) ? sigma_kernel<T>(zl, S.nzl, zr, S.nzr, z, nz) : T(0);
+        c.szm = sigma_mesh<T>(zl, S.nzl, zr, S.nzr, z, nz);
     }
     c.img = (MODE == 2) && (k % B.imaging_period == 0) && B.dw;
     const bool store = (MODE == 1) && (k % B.imaging_period == 0) && B.dw;
-    const long long soff = (long long)(slot - seg_base) * B.total_n + off;
-    const bool dwin = (c.img || MODE == 3) && B.dw;
-    c.dwi = make_rsrc(dwin ? (const T *)B.dw + soff : nullptr, dwin ? nbytes : 0);
+    const long long slot_el = (long long)(slot - seg_base) * B.total_n;
+    const long long slot_bytes = B.total_n * SZ;
+    const bool dwon = ((c.img || MODE == 3) && B.dw) || store;
+    c.dw = make_rsrc(dwon ? (T *)B.dw + slot_el : nullptr, dwon ? slot_bytes : 0);
     c.has_dwo = store;
-    c.dwo = make_rsrc(store ? (T *)B.dw + soff : nullptr, store ? nbytes : 0);
     c.has_fst = B.fstore != nullptr;
-    c.fst = make_rsrc(c.has_fst ? (T *)B.fstore + soff : nullptr, c.has_fst ? nbytes : 0);
-    c.ic = make_rsrc(MODE == 2 ? B.ic + off : nullptr,
-                     MODE == 2 ? (long long)c.nx * c.nz * (long long)sizeof(double) : 0);
-    c.m1 = make_rsrc(MODE == 3 ? B.born_m1 : nullptr,
-                     MODE == 3 ? (long long)(S.mx_hi + 1) * S.mnz * (long long)sizeof(T) : 0);
+    c.fst = make_rsrc(c.has_fst ? (T *)B.fstore + slot_el : nullptr, c.has_fst ? slot_bytes : 0);
+    c.ic = make_rsrc(MODE == 2 ? B.ic : nullptr, MODE == 2 ? B.total_n * 8 : 0);
+    c.m1 = make_rsrc(MODE == 3 ? B.born_m1 : nullptr, MODE == 3 ? (long long)(S.mx_hi + 1) * S.mnz * SZ : 0);
 
     // sources of this tile, gathered into LDS (row-major TX x 64): one lane
     // per destination node, each summing its entries in the reference's order
@@ -324,8 +415,7 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
             wave_sync();
             for (int n = n0 + lane; n < n1; n += 64) {
                 double acc = 0.0;
-                for (int e = B.inj_nptr[n]; e < B.inj_nptr[n + 1]; ++e)
-                    acc += B.inj_w[e] * amp_k[B.inj_chan[e]];
+                for (int e = B.inj_nptr[n]; e < B.inj_nptr[n + 1]; ++e) acc += B.inj_w[e] * amp_k[B.inj_chan[e]];
                 lds_f[B.inj_node[n]] = (T)acc;
             }
             wave_sync();
@@ -336,11 +426,11 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
     if (full) {
         constexpr int NB = block_rows<T, MODE, true>();
 #pragma unroll 1
-        for (int jb = 0; jb < TX; jb += NB) rows2d<T, R, NB, MODE, true>(c, P, x0 + jb, jb);
+        for (int jb = 0; jb < TX; jb += NB) rows_full<T, R, NB, MODE, PACKED>(c, P, x0 + jb, jb);
     } else {
         constexpr int NB = block_rows<T, MODE, false>();
 #pragma unroll 1
-        for (int jb = 0; jb < TX; jb += NB) rows2d<T, R, NB, MODE, false>(c, P, x0 + jb, jb);
+        for (int jb = 0; jb < TX; jb += NB) rows_int<T, R, NB, MODE, PACKED>(c, P, x0 + jb, jb);
     }
 }
 
@@ -375,6 +465,7 @@ __global__ __launch_bounds__(256) void k_step2d(const pysit_amd_batch B, const S
                                                 int n_rec_lanes, int rec_blocks)
 {
     __shared__ T lds_f[4][Tile2D<T>::TX * 64];
+    __shared__ T lds_z[4][4 * zrow_len<R>()];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int b = blockIdx.x;
@@ -387,10 +478,10 @@ __global__ __launch_bounds__(256) void k_step2d(const pysit_amd_batch B, const S
     const int gw = xcd_block(b - rec_blocks, gridDim.x - rec_blocks) * 4 + w;
     if (gw >= n_tile_waves) return;
     if (SINGLE) {
-        tile2d<T, R, MODE>(B, P, P.shot0, gw, 1, gw, slot, seg_base, lane, lds_f[w]);
+        tile2d<T, R, MODE, SINGLE>(B, P, P.shot0, gw, 1, gw, slot, seg_base, lane, lds_f[w], lds_z[w]);
     } else {
         const int4 td = reinterpret_cast<const int4 *>(B.tiles)[gw];
-        tile2d<T, R, MODE>(B, P, B.shots[td.x], gw, td.z, td.y, slot, seg_base, lane, lds_f[w]);
+        tile2d<T, R, MODE, SINGLE>(B, P, B.shots[td.x], gw, td.z & 1, td.y, slot, seg_base, lane, lds_f[w], lds_z[w]);
     }
 }
 
@@ -403,12 +494,18 @@ static int launch2d(const pysit_amd_batch &B, const Step2DParams<T> &P, int slot
     const int rec_blocks = recs ? ((recs + 15) / 16 + 7) / 8 * 8 : 0;
     const int tile_blocks = (n_tiles + 3) / 4;
     if (rec_blocks + tile_blocks == 0) return PYSIT_AMD_OK;
-    if (P.single)
-        hipLaunchKernelGGL((k_step2d<T, R, MODE, true>), dim3(rec_blocks + tile_blocks), dim3(256), 0, st,
-                           B, P, slot, seg_base, n_tiles, recs, rec_blocks);
-    else
+    if (P.single) {
+        if constexpr (MODE == 0) {
+            hipLaunchKernelGGL((k_step2d<T, R, MODE, true>), dim3(rec_blocks + tile_blocks), dim3(256), 0, st,
+                               B, P, slot, seg_base, n_tiles, recs, rec_blocks);
+        } else {
+            set_error("the single-shot step only propagates");
+            return PYSIT_AMD_EINVAL;
+        }
+    } else {
         hipLaunchKernelGGL((k_step2d<T, R, MODE, false>), dim3(rec_blocks + tile_blocks), dim3(256), 0, st,
                            B, P, slot, seg_base, n_tiles, recs, rec_blocks);
+    }
     PA_HIP(hipGetLastError());
     return PYSIT_AMD_OK;
 }
@@ -443,6 +540,27 @@ int run2d(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int 
     P.rhs = nullptr;
     P.single = 0;
     P.general_phi = 0;
+    // the six state buffers share one resource: they must lie in one region
+    {
+        const char *f[6] = {(const char *)B.u[0], (const char *)B.u[1], (const char *)B.phix[0],
+                            (const char *)B.phix[1], (const char *)B.phiz[0], (const char *)B.phiz[1]};
+        const char *lo = f[0], *hi = f[0];
+        for (int i = 0; i < 6; ++i) {
+            PA_CHECK(f[i] != nullptr, "2D batch state buffers (u, phix, phiz) must all be given");
+            lo = f[i] < lo ? f[i] : lo;
+            hi = f[i] > hi ? f[i] : hi;
+        }
+        const long long ext = (long long)(hi - lo) + B.total_n * (long long)sizeof(T);
+        PA_CHECK(ext <= kMaxRsrcBytes, "2D batch state buffers must lie in one region of at most " +
+                                           std::to_string(kMaxRsrcBytes) + " bytes");
+        for (int i = 0; i < 6; ++i) {
+            PA_CHECK((f[i] - lo) % (long long)sizeof(T) == 0, "misaligned 2D batch state buffer");
+            P.fo[i / 2][i % 2] = (int)(f[i] - lo);
+        }
+        P.state = lo;
+        P.state_bytes = (int)ext;
+    }
+    PA_CHECK(B.total_n * 8 <= kMaxRsrcBytes, "batch too large for one launch (total_n)");
     const int n = slot_end - slot_begin;
     for (int i = 0; i < n; ++i) {
         const int j = direction > 0 ? slot_begin + i : slot_end - 1 - i;
@@ -482,6 +600,7 @@ int step2d_entry(int order, const T *km1_u, const T *k_phix, const T *k_phiz, co
     PA_CHECK(kp1_u != km1_u && kp1_u != k_u && kp1_phix != k_phix && kp1_phiz != k_phiz,
              "outputs must not alias inputs");
     const long long N = (long long)nx * nz;
+    PA_CHECK(N * (long long)sizeof(T) <= kMaxRsrcBytes, "grid too large for one step launch");
     // the kernel updates u in place over u_{k-1}: seed the output with it
     PA_HIP(hipMemcpyAsync(kp1_u, km1_u, N * sizeof(T), hipMemcpyDeviceToDevice, st));
 

@@ -106,44 +106,44 @@ __device__ __forceinline__ void wave_sync()
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // ---- raw buffer access -----------------------------------------------------
-// A buffer resource covers `bytes` from `base`; loads at a byte offset beyond
-// it return 0 and stores there are dropped (hardware range check), so masked
-// lanes need no branches: they pass kOOB.  Resources are wave-uniform
-// (SGPRs).  Offsets are 32-bit, so one resource spans at most 2 GiB; callers
-// base them per shot.
+// A buffer resource covers `bytes` from `base`.  An access names a per-lane
+// byte offset (VGPR) and a wave-uniform byte offset (SGPR, e.g. the row and
+// field); a lane whose per-lane offset is kOOB lies beyond every resource, so
+// its load returns 0 and its store is dropped by the hardware range check:
+// masked lanes need no branches.  Resources are wave-uniform (SGPRs).
+// Resource sizes stay below kOOB (kMaxRsrcBytes); the host checks that every
+// region it hands to a kernel fits.
 typedef __amdgpu_buffer_rsrc_t Rsrc;
 constexpr int kOOB = 0x7ffffff0;
+constexpr long long kMaxRsrcBytes = 0x7fff0000LL;
 constexpr int kRsrcWord3 = 0x00020000;  // gfx9-family raw buffer, 32-bit data format
 
 __device__ __forceinline__ Rsrc make_rsrc(const void *base, long long bytes)
 {
-    const int n = bytes > 0x7fffffffLL ? 0x7fffffff : (int)bytes;
+    // the size is wave-uniform; readfirstlane keeps the clamp's result (and so
+    // the whole resource) in SGPRs, where buffer instructions need it
+    const int n = __builtin_amdgcn_readfirstlane(
+        bytes > kMaxRsrcBytes ? (int)kMaxRsrcBytes : (bytes < 0 ? 0 : (int)bytes));
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, n, kRsrcWord3);
 }
 
-template <typename T> __device__ __forceinline__ T buf_ld(Rsrc r, int off);
-template <> __device__ __forceinline__ float buf_ld<float>(Rsrc r, int off)
+template <typename T> __device__ __forceinline__ T buf_ld(Rsrc r, int voff, int soff);
+template <> __device__ __forceinline__ float buf_ld<float>(Rsrc r, int voff, int soff)
 {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
-template <> __device__ __forceinline__ double buf_ld<double>(Rsrc r, int off)
+template <> __device__ __forceinline__ double buf_ld<double>(Rsrc r, int voff, int soff)
 {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
-__device__ __forceinline__ void buf_st(Rsrc r, int off, float v)
+__device__ __forceinline__ void buf_st(Rsrc r, int voff, int soff, float v)
 {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, 0);
 }
-__device__ __forceinline__ void buf_st(Rsrc r, int off, double v)
+__device__ __forceinline__ void buf_st(Rsrc r, int voff, int soff, double v)
 {
     typedef unsigned int u2 __attribute__((ext_vector_type(2)));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, off, 0, 0);
-}
-// byte offset of element idx of T when `ok`, else out of range
-template <typename T>
-__device__ __forceinline__ int boff(bool ok, long long idx)
-{
-    return ok ? (int)(idx * (long long)sizeof(T)) : kOOB;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, soff, 0);
 }
 
 // Kernel-side PML profile value at node i of an axis of n nodes (reference

@@ -31,6 +31,21 @@ TILE_Z = 64      # z nodes per wavefront (the contiguous axis)
 TILE_Y3 = 8      # y rows per 3D workgroup
 
 
+def z_tiles_2d(nz):
+    """2D z tiles: (start, owned end) per tile.  Tiles start at 0, 64, 128, ...
+    except the last, which ends on node nz-1; each node is owned by exactly one
+    tile (the kernel's tiling, cda2d.hip tile2d)."""
+    ntz = -(-nz // TILE_Z)
+    zlast = max(0, nz - TILE_Z)
+    out = []
+    for tz in range(ntz):
+        if tz == ntz - 1:
+            out.append((zlast, nz))
+        else:
+            out.append((tz * TILE_Z, min(tz * TILE_Z + TILE_Z, zlast)))
+    return out
+
+
 def _dev():
     if not torch.cuda.is_available():
         raise N.NativeError("no ROCm device visible: the pysit_amd solver path runs on the GPU only")
@@ -151,17 +166,20 @@ class DeviceBatch:
         """(ntx, nty, ntz) and the per-tile PML flag array (tloc order)."""
         R = self.R
         if self.dim == 2:
+            # a tile takes the plain path only if no node within R of the
+            # nodes it owns is a PML node, an edge node or outside the grid
             nx, nz = s.shape
-            ntz = -(-nz // TILE_Z)
+            zt = z_tiles_2d(nz)
+            ntz = len(zt)
             ntx = -(-nx // self.tile_x)
             flags = np.zeros(ntx * ntz, dtype=np.int32)
             (xl, xr), (zl, zr) = s.kpml
             for tx in range(ntx):
                 x0, x1 = tx * self.tile_x, min(nx, (tx + 1) * self.tile_x) - 1
-                xp = (x0 - R < xl.size) or (x1 + R >= nx - xr.size)
-                for tz in range(ntz):
-                    z0, z1 = tz * TILE_Z, min(nz, (tz + 1) * TILE_Z) - 1
-                    zp = (z0 - R < zl.size) or (z1 + R >= nz - zr.size)
+                xp = (x0 - R < max(xl.size, 1)) or (x1 + R > nx - 1 - max(xr.size, 1))
+                for tz, (z0, zhi) in enumerate(zt):
+                    z1 = zhi - 1
+                    zp = (z0 - R < max(zl.size, 1)) or (z1 + R > nz - 1 - max(zr.size, 1))
                     flags[tx * ntz + tz] = 1 if (xp or zp) else 0
             return (ntx, 1, ntz), flags
         nx, ny, nz = s.shape
@@ -182,11 +200,19 @@ class DeviceBatch:
                     flags[(tx * nty + ty) * ntz + tz] = 1 if (xp or yp or zp) else 0
         return (ntx, nty, ntz), flags
 
-    def _tile_of_pos(self, s_tiles, pos):
+    def _tile_of_pos(self, s, s_tiles, pos):
+        """(tile within the shot, position within the tile) of grid nodes."""
         ntx, nty, ntz = s_tiles
         if self.dim == 2:
-            return (pos[:, 0] // self.tile_x) * ntz + pos[:, 1] // TILE_Z
-        return ((pos[:, 0] // self.tile_x) * nty + pos[:, 1] // TILE_Y3) * ntz + pos[:, 2] // TILE_Z
+            nz = s.shape[1]
+            zlast = max(0, nz - TILE_Z)
+            z = pos[:, 1]
+            tz = np.where(z >= zlast, ntz - 1, z // TILE_Z)
+            z0 = np.where(tz == ntz - 1, zlast, tz * TILE_Z)
+            return (pos[:, 0] // self.tile_x) * ntz + tz, (pos[:, 0] % self.tile_x) * TILE_Z + (z - z0)
+        t = ((pos[:, 0] // self.tile_x) * nty + pos[:, 1] // TILE_Y3) * ntz + pos[:, 2] // TILE_Z
+        local = ((pos[:, 0] % self.tile_x) * TILE_Y3 + pos[:, 1] % TILE_Y3) * TILE_Z + pos[:, 2] % TILE_Z
+        return t, local
 
     def _build_tables(self):
         dev = self.dev
@@ -284,12 +310,8 @@ class DeviceBatch:
                 continue
             if np.any(pos < 0) or np.any(pos >= np.asarray(s.shape)[None, :]):
                 raise ValueError("injection entry outside the padded grid")
-            tloc = self._tile_of_pos(self.tiles_info[si][0], pos)
+            tloc, local = self._tile_of_pos(s, self.tiles_info[si][0], pos)
             tglob = np.array([self._tile_index[(si, int(t))] for t in tloc], dtype=np.int64)
-            if self.dim == 2:
-                local = (pos[:, 0] % self.tile_x) * TILE_Z + pos[:, 1] % TILE_Z
-            else:
-                local = ((pos[:, 0] % self.tile_x) * TILE_Y3 + pos[:, 1] % TILE_Y3) * TILE_Z + pos[:, 2] % TILE_Z
             node_tile.append(tglob)
             node_pos.append(local)
             e_chan.append(chan)
@@ -363,7 +385,11 @@ class DeviceBatch:
         return ("u", "phix", "phiz") if self.dim == 2 else ("u", "phix", "phiy", "phiz", "psi")
 
     def _new_state(self):
-        return {f: torch.zeros((2, self.total_n), dtype=self.tdtype, device=self.dev) for f in self._fields()}
+        """Both time levels of every field, in one allocation (the 2D kernel
+        reaches all of them through one buffer resource)."""
+        fields = self._fields()
+        block = torch.zeros((2 * len(fields), self.total_n), dtype=self.tdtype, device=self.dev)
+        return {f: block[2 * i:2 * i + 2] for i, f in enumerate(fields)}
 
     def _alloc_state(self):
         self.fstate = self._new_state()
