@@ -33,6 +33,11 @@ const char *pysit_amd_last_error(void);
 /* Library version string. */
 const char *pysit_amd_version(void);
 
+/* Tile geometry the library was built with: rows along x (axis 0) or y
+ * (axis 1, 3D) of one tile, for dim 2 or 3 and dtype 0 (f32) or 1 (f64).
+ * pysit_amd_batch.tile_x / tile_y must carry these values. */
+int pysit_amd_tile_rows(int dim, int dtype, int axis);
+
 /* ABI self-check for bindings: returns 0 when the caller's sizes of
  * pysit_amd_shot and pysit_amd_batch match the library's, else -1. */
 int pysit_amd_abi_check(int shot_size, int batch_size);

@@ -23,6 +23,7 @@ FLAGS = [
     "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
+    "-Werror=uninitialized",
     "-Wno-unused-function",
 ]
 

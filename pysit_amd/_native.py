@@ -60,6 +60,7 @@ _STEP2D = [_i] + [_vp] * 6 + [_vp, _i] * 4 + [_d] * 3 + [_i] * 2 + [_vp] * 3 + [
 _STEP3D = [_i] + [_vp] * 8 + [_vp, _i] * 6 + [_d] * 4 + [_i] * 3 + [_vp] * 5 + [_vp]
 SIGNATURES = {
     "pysit_amd_abi_check": [_i, _i],
+    "pysit_amd_tile_rows": [_i, _i, _i],
     "pysit_amd_cda_step2d_f32": _STEP2D,
     "pysit_amd_cda_step2d_f64": _STEP2D,
     "pysit_amd_cda_step3d_f32": _STEP3D,
@@ -90,6 +91,14 @@ def lib():
             raise NativeError(L.pysit_amd_last_error().decode())
         _lib = L
     return _lib
+
+
+def tile_rows(dim, dtype_code, axis=0):
+    """Tile rows along x (axis 0) or y (axis 1) the library was built with."""
+    n = lib().pysit_amd_tile_rows(dim, dtype_code, axis)
+    if n <= 0:
+        raise NativeError(lib().pysit_amd_last_error().decode())
+    return n
 
 
 def call(name, *args):

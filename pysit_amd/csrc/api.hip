@@ -14,6 +14,9 @@ int run2d(const pysit_amd_batch &, int, int, int, int, int, const int *, const i
 template <typename T>
 int run3d(const pysit_amd_batch &, int, int, int, int, int, const int *, const int *, hipStream_t);
 
+int tile_rows_2d(int dtype);
+int tile_rows_3d(int axis);
+
 template <typename T>
 int step2d_entry(int, const T *, const T *, const T *, const T *, const T *, const T *, const T *,
                  int, const T *, int, const T *, int, const T *, int, double, double, double, int,
@@ -90,6 +93,17 @@ extern "C" {
 const char *pysit_amd_last_error(void) { return g_err.c_str(); }
 
 const char *pysit_amd_version(void) { return "pysit_amd 0.1 (gfx950)"; }
+
+int pysit_amd_tile_rows(int dim, int dtype, int axis)
+{
+    clear_error();
+    if ((dim != 2 && dim != 3) || (dtype != 0 && dtype != 1) || axis < 0 || axis > 1) {
+        set_error("tile_rows: dim must be 2 or 3, dtype 0 or 1, axis 0 (x) or 1 (y)");
+        return PYSIT_AMD_EINVAL;
+    }
+    if (dim == 2) return axis == 0 ? pa::tile_rows_2d(dtype) : 1;
+    return pa::tile_rows_3d(axis);
+}
 
 int pysit_amd_abi_check(int shot_size, int batch_size)
 {

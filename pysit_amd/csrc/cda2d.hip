@@ -8,21 +8,20 @@
 //
 // Mapping to CDNA: one wavefront owns a tile of 64 consecutive z (the
 // contiguous axis: every row access is one coalesced 256 B transaction) by
-// TX rows along x (16 in FP32, 8 in FP64).  The z tiles of a shot start at
+// TX = 8 rows along x.  The z tiles of a shot start at
 // 0, 64, 128, ... except the last, which ends on the last node; the node
 // ranges the tiles own are disjoint (a tile lane that another tile owns only
 // loads), so the PML band at the far z side costs one PML tile, not two.
-// A tile is processed in register blocks of NB rows: all loads of a block
-// (the NB + 2r rows of u its x derivatives need, u_{k-1}, the model, the
-// imaging accumulator) are issued up front, then the rows are computed
-// fully unrolled.  z derivatives take their neighbours from other lanes through
-// an LDS row per wavefront; lanes 0..r-1 and 64-r..63 also carry the r-node
-// halo beyond each side of the tile.
+// The rows of a tile are computed in order as a software-pipelined march:
+// each u row is loaded once, the loads of the next rows are in flight while
+// the current row is computed.  z derivatives take their neighbours from
+// other lanes through an LDS row per wavefront; lanes 0..r-1 and 64-r..63
+// also carry the r-node halo beyond each side of the tile.
 //
 // Addressing: every array is reached through a raw buffer resource with a
-// per-lane byte offset fixed for the whole tile (z, or an out-of-range value
-// for masked lanes) and a wave-uniform row offset in an SGPR, so the row loop
-// spends no vector instructions on addresses or masks.  In a batch the six
+// per-lane byte offset (z, or an out-of-range value for masked lanes, plus
+// the row) and a wave-uniform base offset in an SGPR, so the row loop spends
+// one vector add per row on addresses and none on masks.  In a batch the six
 // state buffers (u, Phix, Phiz, two time levels each) lie in one region and
 // share one resource.
 //
@@ -49,16 +48,17 @@
 
 namespace pa {
 
+// Tile rows (compile-time tuning knobs; the host reads them back through
+// pysit_amd_tile_rows).
+#ifndef PA2D_TX_F32
+#define PA2D_TX_F32 8
+#endif
+#ifndef PA2D_TX_F64
+#define PA2D_TX_F64 8
+#endif
 template <typename T> struct Tile2D;
-template <> struct Tile2D<float> { static constexpr int TX = 16; };
-template <> struct Tile2D<double> { static constexpr int TX = 8; };
-
-// rows per register block: fewer where more state is live per row
-template <typename T, int MODE, bool FULL>
-constexpr int block_rows()
-{
-    return Tile2D<T>::TX / ((FULL ? 2 : 1) * (MODE >= 2 ? 2 : 1));
-}
+template <> struct Tile2D<float> { static constexpr int TX = PA2D_TX_F32; };
+template <> struct Tile2D<double> { static constexpr int TX = PA2D_TX_F64; };
 
 // state buffers of one step: u_k, u_{k-1} (overwritten by u_{k+1}),
 // Phix_k, Phix_{k+1}, Phiz_k, Phiz_{k+1}
@@ -98,8 +98,7 @@ struct Ctx2D {
     State2D<PACKED> s;
     Rsrc model, m1, dw, fst, ic, rhs;  // dw: read (adjoint, Born) or written (forward)
     bool has_fst, has_dwo, has_rhs, has_inj, img, general_phi;
-    const T *finj;  // LDS row sources of this wave
-    const T *xl, *xr;
+    const T *finj;  // this lane's LDS source column (or a zero slot)
     int nxl, nxr;
     int nx, nz, lane;
     int rowb;                         // bytes per grid row
@@ -110,14 +109,51 @@ struct Ctx2D {
     int hv;    // halo node inside the grid
     int ov;    // owned node: u_{k-1} and every store
     int ov8;   // owned node, FP64 arrays
+    int ovd;   // owned node where the dWaveOp store is read or written this launch
+    int ovi;   // owned node where the imaging accumulator is updated this launch (FP64)
+    int ovf;   // owned node where the field store is written
+    int fstride;  // LDS row stride of the tile's sources: 64, or 0 (all rows read zslot)
     int mv;    // model column (edge padding by clamping)
     int zpv;   // centre node where Phi can be nonzero without a PML row
     int hpv;   // halo node likewise
     bool zedge;
-    T szk, szm;  // lane z profile (kernel, mesh)
+    // PML profiles, each as the two halves of a masked lookup (summed only
+    // after the first rows' loads are issued, so waiting for them costs nothing)
+    T szk2[2], szm2[2];  // lane z profile (kernel, mesh)
+    T vsig2[2];          // x profile of the tile rows: lane r < TX kernel, lane TX + r mesh
     T *zbuf;     // this wave's z-exchange rows (LDS), 4 x ZROW
     int hslot;   // where this lane's halo value goes in a row
 };
+
+// value of lane `l` (wave-uniform l) in an SGPR
+__device__ __forceinline__ float lane_bcast(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_bcast(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// PML profile value at node i (i < 0: none) of an axis of n nodes from the
+// left array l (nl values, by node) and the right array r (nr values): the
+// kernel form takes r one node shifted towards the edge and never covers the
+// edge node itself (constant_density_acoustic_time_scalar_2D.h:137-157), the
+// mesh form lays r on the last nr nodes (StructuredPML, mesh.py:770-828).
+// Two masked buffer loads, at most one in range: the value is their sum.
+template <typename T>
+__device__ __forceinline__ void sigma_lane(Rsrc l, int nl, Rsrc r, int nr, int i, int n, bool kernel, T (&out)[2])
+{
+    constexpr int SZ = sizeof(T);
+    const bool inl = i >= 0 && i < nl;
+    const bool inr = !inl && i >= 0 && i >= n - nr && (!kernel || i <= n - 2);
+    const int ri = kernel ? nr - ((n - 1) - i) : i - (n - nr);
+    out[0] = buf_ld<T>(l, inl ? i * SZ : kOOB, 0);
+    out[1] = buf_ld<T>(r, inr ? ri * SZ : kOOB, 0);
+}
 
 // z-exchange row: slots 0..R-1 left halo, R..R+63 the lanes, R+64..2R+63
 // right halo, then a parking slot per lane for lanes that hold no halo
@@ -144,35 +180,55 @@ __device__ __forceinline__ void z_taps(const Ctx2D<T, R, PACKED> &c, int buf, T 
     }
 }
 
-// Interior rows: no PML node within r, no edge node, every row in the grid.
-template <typename T, int R, int NB, int MODE, bool PACKED>
-__device__ __forceinline__ void rows_int(const Ctx2D<T, R, PACKED> &c, const Step2DParams<T> &P, int xb, int jb)
+// Rows of a tile are computed in order (a march along x).  Each u row is
+// loaded once per tile; the loads of the rows `PA2D_PREFETCH` ahead are
+// issued before the current row is computed, so every wave keeps a steady
+// stream of loads in flight while it computes.  Row byte offsets go into the
+// per-lane offset (one vector add per row), the buffer's base offset stays in
+// an SGPR per array.
+#ifndef PA2D_PREFETCH
+#define PA2D_PREFETCH 2
+#endif
+
+// Plain tiles: no PML node within r of an owned node, no edge node, every
+// row of the march inside the grid.
+template <typename T, int R, int MODE, bool PACKED>
+__device__ __forceinline__ void march_int(const Ctx2D<T, R, PACKED> &c, const Step2DParams<T> &P, int x0)
 {
+    constexpr int TX = Tile2D<T>::TX;
+    constexpr int D = PA2D_PREFETCH;
+    constexpr int NU = TX + 2 * R;  // u rows x0-R .. x0+TX+R-1
     const int rb = c.rowb;
-    T uc[NB + 2 * R];
+    const int so_u = c.s.so(F_U), so_uo = c.s.so(F_UO);
+    T u[NU], uh[TX], um[TX], cv[TX], fv[TX], dv[TX], mv[TX];
+    double icv[TX];
+    const int zv = c.zv, hv = c.hv, ov = c.ov, ovd = c.ovd, ovi = c.ovi, ovf = c.ovf, mvo = c.mv;
+#define PA_LD_U(s_) u[s_] = buf_ld<T>(c.s.rs(F_U), zv + (x0 - R + (s_)) * rb, so_u)
+#define PA_LD_ROW(r_)                                                                               \
+    do {                                                                                            \
+        const int r = (r_), x = x0 + r, xr = x * rb;                                                \
+        uh[r] = buf_ld<T>(c.s.rs(F_U), hv + xr, so_u);                                              \
+        um[r] = buf_ld<T>(c.s.rs(F_UO), ov + xr, so_uo);                                            \
+        const int mr = clampi(x - c.mx_off, c.mx_lo, c.mx_hi) * c.mrowb;                           \
+        cv[r] = buf_ld<T>(c.model, mvo, mr);                                                        \
+        fv[r] = c.finj[r * c.fstride];                                                              \
+        dv[r] = (MODE >= 2) ? buf_ld<T>(c.dw, ovd + xr, c.shb) : T(0);                              \
+        mv[r] = (MODE == 3) ? buf_ld<T>(c.m1, mvo, mr) : T(0);                                      \
+        icv[r] = (MODE == 2) ? buf_ld<double>(c.ic, ovi + xr * (8 / (int)sizeof(T)), c.shb8) : 0.0; \
+    } while (0)
 #pragma unroll
-    for (int j = 0; j < NB + 2 * R; ++j) uc[j] = buf_ld<T>(c.s.rs(F_U), c.zv, c.s.so(F_U) + (xb - R + j) * rb);
-    T uh[NB], um[NB], cv[NB], fv[NB], dv[NB], mv[NB];
-    double icv[NB];
-    const bool rd = MODE == 3 || (MODE == 2 && c.img);
+    for (int q = 0; q < 2 * R + D && q < NU; ++q) PA_LD_U(q);
 #pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        const int x = xb + j;
+    for (int q = 0; q < D && q < TX; ++q) PA_LD_ROW(q);
+    __builtin_amdgcn_sched_barrier(0);  // the prologue's loads go out before any row work
+#pragma unroll
+    for (int j = 0; j < TX; ++j) {
+        if (j + 2 * R + D < NU) PA_LD_U(j + 2 * R + D);
+        if (j + D < TX) PA_LD_ROW(j + D);
+        __builtin_amdgcn_sched_barrier(0);  // prefetch issued before this row is computed
+        const int x = x0 + j;
         const int xr = x * rb;
-        uh[j] = buf_ld<T>(c.s.rs(F_U), c.hv, c.s.so(F_U) + xr);
-        um[j] = buf_ld<T>(c.s.rs(F_UO), c.ov, c.s.so(F_UO) + xr);
-        const int mr = clampi(x - c.mx_off, c.mx_lo, c.mx_hi) * c.mrowb;
-        cv[j] = buf_ld<T>(c.model, c.mv, mr);
-        fv[j] = c.has_inj ? c.finj[(jb + j) * 64 + c.lane] : T(0);
-        dv[j] = rd ? buf_ld<T>(c.dw, c.ov, c.shb + xr) : T(0);
-        mv[j] = (MODE == 3) ? buf_ld<T>(c.m1, c.mv, mr) : T(0);
-        icv[j] = (MODE == 2 && c.img) ? buf_ld<double>(c.ic, c.ov8, c.shb8 + x * c.nz * 8) : 0.0;
-    }
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        const int x = xb + j;
-        const int xr = x * rb;
-        const T v = uc[R + j];
+        const T v = u[R + j];
         T f = fv[j];
         if (MODE == 3) f += mv[j] * (-dv[j]);  // Born source: m1_pad * (-dWaveOp0_k)
         T zp[R + 1], zm[R + 1];
@@ -180,7 +236,7 @@ __device__ __forceinline__ void rows_int(const Ctx2D<T, R, PACKED> &c, const Ste
         T lx = T(0), lz = T(0);
 #pragma unroll
         for (int i = R; i >= 1; --i) {
-            lx = fma(P.cx.d2[i], uc[R + j + i] + uc[R + j - i], lx);
+            lx = fma(P.cx.d2[i], u[R + j + i] + u[R + j - i], lx);
             lz = fma(P.cz.d2[i], zp[i] + zm[i], lz);
         }
         lx = fma(P.cx.d2[0], v, lx);
@@ -189,59 +245,78 @@ __device__ __forceinline__ void rows_int(const Ctx2D<T, R, PACKED> &c, const Ste
         const T c2 = cv[j] * cv[j];
         const T src = f + lap;
         const T up = fma(P.tc.dt2, c2 * src, T(2) * v - um[j]);
-        if (MODE == 2 && c.img) buf_st(c.ic, c.ov8, c.shb8 + x * c.nz * 8, icv[j] + (double)v * (double)dv[j]);
-        buf_st(c.s.rs(F_UO), c.ov, c.s.so(F_UO) + xr, up);
-        if (MODE == 1 && c.has_dwo) buf_st(c.dw, c.ov, c.shb + xr, c2 * src);
-        if (c.has_fst) buf_st(c.fst, c.ov, c.shb + xr, v);
+        if (MODE == 2) buf_st(c.ic, ovi + xr * (8 / (int)sizeof(T)), c.shb8, icv[j] + (double)v * (double)dv[j]);
+        buf_st(c.s.rs(F_UO), ov + xr, so_uo, up);
+        if (MODE == 1) buf_st(c.dw, ovd + xr, c.shb, c2 * src);
+        buf_st(c.fst, ovf + xr, c.shb, v);
     }
+#undef PA_LD_U
+#undef PA_LD_ROW
 }
 
-// Rows with the PML update, edges and rows beyond the grid.
-template <typename T, int R, int NB, int MODE, bool PACKED>
-__device__ __forceinline__ void rows_full(const Ctx2D<T, R, PACKED> &c, const Step2DParams<T> &P, int xb, int jb)
+// PML tiles: the full update, edges, rows beyond the grid.
+template <typename T, int R, int MODE, bool PACKED>
+__device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const Step2DParams<T> &P, int x0)
 {
+    constexpr int TX = Tile2D<T>::TX;
+    constexpr int D = PA2D_PREFETCH;
+    constexpr int NU = TX + 2 * R;
     const int rb = c.rowb, nx = c.nx;
-    T uc[NB + 2 * R], pc[NB + 2 * R];
+    const int so_u = c.s.so(F_U), so_uo = c.s.so(F_UO), so_px = c.s.so(F_PX), so_pz = c.s.so(F_PZ);
+    // values, not fields: a conditional on two fields would select their
+    // addresses and keep the context in memory
+    const int zv = c.zv, zpv = c.zpv, hv = c.hv, hpv = c.hpv, ov = c.ov, ovd = c.ovd, ovi = c.ovi, ovf = c.ovf;
+    T u[NU], px[NU], uh[TX], um[TX], cv[TX], fv[TX], dv[TX], mv[TX], pv[TX], ph[TX];
+    double icv[TX];
+    const int mvo = c.mv;
+#define PA_LD_U(s_)                                                                                 \
+    do {                                                                                            \
+        const int xx = x0 - R + (s_);                                                               \
+        const bool rin = xx >= 0 && xx < nx;                                                        \
+        const bool xp = xx < c.nxl || xx >= nx - c.nxr;                                             \
+        const int xr = (rin ? xx : 0) * rb;                                                         \
+        u[s_] = buf_ld<T>(c.s.rs(F_U), rin ? zv + xr : kOOB, so_u);                                 \
+        px[s_] = buf_ld<T>(c.s.rs(F_PX), rin ? (xp ? zv : zpv) + xr : kOOB, so_px);                 \
+    } while (0)
+#define PA_LD_ROW(r_)                                                                               \
+    do {                                                                                            \
+        const int r = (r_), x = x0 + r;                                                             \
+        const bool own = x < nx;                                                                    \
+        const int xs = own ? x : nx - 1;                                                            \
+        const int xr = xs * rb;                                                                     \
+        const bool xp = x < c.nxl || x >= nx - c.nxr;                                               \
+        const int ovr = own ? ov + xr : kOOB;                                                       \
+        uh[r] = buf_ld<T>(c.s.rs(F_U), own ? hv + xr : kOOB, so_u);                                 \
+        um[r] = buf_ld<T>(c.s.rs(F_UO), ovr, so_uo);                                                \
+        const int mr = clampi(xs - c.mx_off, c.mx_lo, c.mx_hi) * c.mrowb;                          \
+        cv[r] = buf_ld<T>(c.model, mvo, mr);                                                        \
+        T f = buf_ld<T>(c.rhs, ovr, 0); /* zero-sized resource without a dense rhs */               \
+        f += c.finj[r * c.fstride];                                                                 \
+        fv[r] = f;                                                                                  \
+        dv[r] = (MODE >= 2) ? buf_ld<T>(c.dw, own ? ovd + xr : kOOB, c.shb) : T(0);                 \
+        mv[r] = (MODE == 3) ? buf_ld<T>(c.m1, mvo, mr) : T(0);                                      \
+        icv[r] = (MODE == 2) ? buf_ld<double>(c.ic, own ? ovi + xr * (8 / (int)sizeof(T)) : kOOB, c.shb8) : 0.0; \
+        pv[r] = buf_ld<T>(c.s.rs(F_PZ), own ? (xp ? zv : zpv) + xr : kOOB, so_pz);                  \
+        ph[r] = buf_ld<T>(c.s.rs(F_PZ), own ? (xp ? hv : hpv) + xr : kOOB, so_pz);                  \
+    } while (0)
 #pragma unroll
-    for (int j = 0; j < NB + 2 * R; ++j) {
-        const int xx = xb - R + j;
-        const bool rin = xx >= 0 && xx < nx;
-        const bool xp = xx < c.nxl || xx >= nx - c.nxr;
-        const int xr = (rin ? xx : 0) * rb;
-        uc[j] = buf_ld<T>(c.s.rs(F_U), rin ? c.zv : kOOB, c.s.so(F_U) + xr);
-        pc[j] = buf_ld<T>(c.s.rs(F_PX), rin ? (xp ? c.zv : c.zpv) : kOOB, c.s.so(F_PX) + xr);
-    }
-    T uh[NB], um[NB], cv[NB], fv[NB], dv[NB], mv[NB], pv[NB], ph[NB];
-    double icv[NB];
-    const bool rd = MODE == 3 || (MODE == 2 && c.img);
+    for (int q = 0; q < 2 * R + D && q < NU; ++q) PA_LD_U(q);
 #pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        const int x = xb + j;
+    for (int q = 0; q < D && q < TX; ++q) PA_LD_ROW(q);
+    __builtin_amdgcn_sched_barrier(0);  // the prologue's loads go out before any row work
+    const T szk = c.szk2[0] + c.szk2[1], szm = c.szm2[0] + c.szm2[1];
+    const T vsig = c.vsig2[0] + c.vsig2[1];
+#pragma unroll
+    for (int j = 0; j < TX; ++j) {
+        if (j + 2 * R + D < NU) PA_LD_U(j + 2 * R + D);
+        if (j + D < TX) PA_LD_ROW(j + D);
+        __builtin_amdgcn_sched_barrier(0);  // prefetch issued before this row is computed
+        const int x = x0 + j;
         const bool own = x < nx;
         const int xs = own ? x : nx - 1;
         const int xr = xs * rb;
-        const bool xp = x < c.nxl || x >= nx - c.nxr;
-        const int ovr = own ? c.ov : kOOB;
-        uh[j] = buf_ld<T>(c.s.rs(F_U), own ? c.hv : kOOB, c.s.so(F_U) + xr);
-        um[j] = buf_ld<T>(c.s.rs(F_UO), ovr, c.s.so(F_UO) + xr);
-        const int mr = clampi(xs - c.mx_off, c.mx_lo, c.mx_hi) * c.mrowb;
-        cv[j] = buf_ld<T>(c.model, c.mv, mr);
-        T f = c.has_rhs ? buf_ld<T>(c.rhs, ovr, xr) : T(0);
-        if (c.has_inj) f += c.finj[(jb + j) * 64 + c.lane];
-        fv[j] = f;
-        dv[j] = rd ? buf_ld<T>(c.dw, ovr, c.shb + xr) : T(0);
-        mv[j] = (MODE == 3) ? buf_ld<T>(c.m1, c.mv, mr) : T(0);
-        icv[j] = (MODE == 2 && c.img) ? buf_ld<double>(c.ic, own ? c.ov8 : kOOB, c.shb8 + xs * c.nz * 8) : 0.0;
-        pv[j] = buf_ld<T>(c.s.rs(F_PZ), own ? (xp ? c.zv : c.zpv) : kOOB, c.s.so(F_PZ) + xr);
-        ph[j] = buf_ld<T>(c.s.rs(F_PZ), own ? (xp ? c.hv : c.hpv) : kOOB, c.s.so(F_PZ) + xr);
-    }
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        const int x = xb + j;
-        const bool own = x < nx;
-        const int xr = (own ? x : nx - 1) * rb;
-        const int ovr = own ? c.ov : kOOB;
-        const T v = uc[R + j];
+        const int ovr = own ? ov + xr : kOOB;
+        const T v = u[R + j];
         T f = fv[j];
         if (MODE == 3) f += mv[j] * (-dv[j]);
         T zp[R + 1], zm[R + 1], pzp[R + 1], pzm[R + 1];
@@ -250,7 +325,7 @@ __device__ __forceinline__ void rows_full(const Ctx2D<T, R, PACKED> &c, const St
         T lx = T(0), lz = T(0);
 #pragma unroll
         for (int i = R; i >= 1; --i) {
-            lx = fma(P.cx.d2[i], uc[R + j + i] + uc[R + j - i], lx);
+            lx = fma(P.cx.d2[i], u[R + j + i] + u[R + j - i], lx);
             lz = fma(P.cz.d2[i], zp[i] + zm[i], lz);
         }
         lx = fma(P.cx.d2[0], v, lx);
@@ -263,18 +338,16 @@ __device__ __forceinline__ void rows_full(const Ctx2D<T, R, PACKED> &c, const St
         T dux = T(0), duz = T(0), dpx = T(0), dpz = T(0);
 #pragma unroll
         for (int i = R; i >= 1; --i) {
-            dux = fma(P.cx.d1[i - 1], uc[R + j + i] - uc[R + j - i], dux);
+            dux = fma(P.cx.d1[i - 1], u[R + j + i] - u[R + j - i], dux);
             duz = fma(P.cz.d1[i - 1], zp[i] - zm[i], duz);
-            dpx = fma(P.cx.d1[i - 1], pc[R + j + i] - pc[R + j - i], dpx);
+            dpx = fma(P.cx.d1[i - 1], px[R + j + i] - px[R + j - i], dpx);
             dpz = fma(P.cz.d1[i - 1], pzp[i] - pzm[i], dpz);
         }
         const T src = ((f + lap) + dpx) + dpz;
-        const int xs = own ? x : nx - 1;
-        const T sx = sigma_kernel<T>(c.xl, c.nxl, c.xr, c.nxr, max(0, min(xs, nx - 2)), nx);
-        const T sxmj = sigma_mesh<T>(c.xl, c.nxl, c.xr, c.nxr, xs, nx);
-        const T szk = c.szk, szm = c.szm;
+        const T sx = lane_bcast(vsig, j);
+        const T sxmj = lane_bcast(vsig, TX + j);
         const T dt = P.tc.dt;
-        const T pxc = pc[R + j], pzc = pv[j];
+        const T pxc = px[R + j], pzc = pv[j];
         const bool pml = sx != T(0) || szk != T(0);
         // damped update (PML node) and plain update, selected per lane
         const T pxd = (pxc - dt * sx * pxc) + dt * (szk - sx) * dux;
@@ -298,15 +371,17 @@ __device__ __forceinline__ void rows_full(const Ctx2D<T, R, PACKED> &c, const St
             dw = T(0);
             phi_store = true;
         }
-        if (MODE == 2 && c.img)
-            buf_st(c.ic, own ? c.ov8 : kOOB, c.shb8 + (own ? x : nx - 1) * c.nz * 8, icv[j] + (double)v * (double)dv[j]);
-        buf_st(c.s.rs(F_UO), ovr, c.s.so(F_UO) + xr, up);
+        if (MODE == 2)
+            buf_st(c.ic, own ? ovi + xr * (8 / (int)sizeof(T)) : kOOB, c.shb8, icv[j] + (double)v * (double)dv[j]);
+        buf_st(c.s.rs(F_UO), ovr, so_uo, up);
         const int op = phi_store ? ovr : kOOB;
-        buf_st(c.s.rs(F_PXO), op, c.s.so(F_PXO) + xr, pxn);
-        buf_st(c.s.rs(F_PZO), op, c.s.so(F_PZO) + xr, pzn);
-        if (MODE == 1 && c.has_dwo) buf_st(c.dw, ovr, c.shb + xr, dw);
-        if (c.has_fst) buf_st(c.fst, ovr, c.shb + xr, v);
+        buf_st(c.s.rs(F_PXO), op, c.s.so(F_PXO), pxn);
+        buf_st(c.s.rs(F_PZO), op, c.s.so(F_PZO), pzn);
+        if (MODE == 1) buf_st(c.dw, own ? ovd + xr : kOOB, c.shb, dw);
+        buf_st(c.fst, own ? ovf + xr : kOOB, c.shb, v);
     }
+#undef PA_LD_U
+#undef PA_LD_ROW
 }
 
 template <typename T, int R, int MODE, bool SINGLE>
@@ -349,6 +424,9 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
     // which buffer holds time level k: parity of the steps this propagation took
     const int taken = (MODE == 2) ? (S.nt - 1 - k) : k;
     const int cur = taken & 1;
+    // disabled arrays get an empty resource on a valid base: every access is
+    // out of range, loads return 0 and stores are dropped
+    const void *vbase = B.u[0];
     c.rowb = nz * SZ;
     c.shb = (int)(S.off * SZ);
     c.shb8 = (int)(S.off * 8);
@@ -375,37 +453,54 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
     c.mx_lo = S.mx_lo;
     c.mx_hi = S.mx_hi;
     c.mrowb = S.mnz * SZ;
-    c.xl = (const T *)S.xl;
-    c.xr = (const T *)S.xr;
     c.nxl = S.nxl;
     c.nxr = S.nxr;
     c.general_phi = P.general_phi;
     c.has_rhs = P.rhs != nullptr;
-    c.rhs = make_rsrc(P.rhs, c.has_rhs ? (long long)nx * nz * SZ : 0);
+    c.rhs = make_rsrc(c.has_rhs ? (const void *)P.rhs : vbase, c.has_rhs ? (long long)nx * nz * SZ : 0);
     c.zbuf = zbuf;
     c.hslot = lane < R ? R + 64 + lane : (lane >= 64 - R ? lane - (64 - R) : 2 * R + 64 + lane);
-    const T *zl = (const T *)S.zl, *zr = (const T *)S.zr;
-    c.szk = c.szm = T(0);
-    if (full && zin) {
-        c.szk = (z <= nz - 2) ? sigma_kernel<T>(zl, S.nzl, zr, S.nzr, z, nz) : T(0);
-        c.szm = sigma_mesh<T>(zl, S.nzl, zr, S.nzr, z, nz);
+    // PML profiles as masked buffer loads (no branches, so the loads stay
+    // ahead of the march in the wait-count order): lane z profile, and the x
+    // profile of the tile's rows, one lane per value (lane r < TX kernel
+    // form, lane TX + r mesh form), read back per row with readlane; rows
+    // past the grid repeat the last row
+    static_assert(2 * TX <= 64, "the x profile of a tile must fit one wavefront");
+    {
+        const Rsrc rzl = make_rsrc(S.nzl ? S.zl : vbase, (long long)S.nzl * SZ);
+        const Rsrc rzr = make_rsrc(S.nzr ? S.zr : vbase, (long long)S.nzr * SZ);
+        const Rsrc rxl = make_rsrc(S.nxl ? S.xl : vbase, (long long)S.nxl * SZ);
+        const Rsrc rxr = make_rsrc(S.nxr ? S.xr : vbase, (long long)S.nxr * SZ);
+        const bool zf = full && zin;
+        sigma_lane<T>(rzl, S.nzl, rzr, S.nzr, zf && z <= nz - 2 ? z : -1, nz, true, c.szk2);
+        sigma_lane<T>(rzl, S.nzl, rzr, S.nzr, zf ? z : -1, nz, false, c.szm2);
+        const int xs = min(x0 + (lane < TX ? lane : lane - TX), nx - 1);
+        const int xi = (full && lane < 2 * TX) ? (lane < TX ? max(0, min(xs, nx - 2)) : xs) : -1;
+        sigma_lane<T>(rxl, S.nxl, rxr, S.nxr, xi, nx, lane < TX, c.vsig2);
     }
     c.img = (MODE == 2) && (k % B.imaging_period == 0) && B.dw;
     const bool store = (MODE == 1) && (k % B.imaging_period == 0) && B.dw;
     const long long slot_el = (long long)(slot - seg_base) * B.total_n;
     const long long slot_bytes = B.total_n * SZ;
     const bool dwon = ((c.img || MODE == 3) && B.dw) || store;
-    c.dw = make_rsrc(dwon ? (T *)B.dw + slot_el : nullptr, dwon ? slot_bytes : 0);
+    c.dw = make_rsrc(dwon ? (const void *)((T *)B.dw + slot_el) : vbase, dwon ? slot_bytes : 0);
     c.has_dwo = store;
     c.has_fst = B.fstore != nullptr;
-    c.fst = make_rsrc(c.has_fst ? (T *)B.fstore + slot_el : nullptr, c.has_fst ? slot_bytes : 0);
-    c.ic = make_rsrc(MODE == 2 ? B.ic : nullptr, MODE == 2 ? B.total_n * 8 : 0);
-    c.m1 = make_rsrc(MODE == 3 ? B.born_m1 : nullptr, MODE == 3 ? (long long)(S.mx_hi + 1) * S.mnz * SZ : 0);
+    c.fst = make_rsrc(c.has_fst ? (const void *)((T *)B.fstore + slot_el) : vbase, c.has_fst ? slot_bytes : 0);
+    c.ic = make_rsrc(MODE == 2 ? (const void *)B.ic : vbase, MODE == 2 ? B.total_n * 8 : 0);
+    c.ovd = dwon ? c.ov : kOOB;
+    c.ovi = c.img ? c.ov8 : kOOB;
+    c.ovf = c.has_fst ? c.ov : kOOB;
+    c.m1 = make_rsrc(MODE == 3 ? B.born_m1 : vbase, MODE == 3 ? (long long)(S.mx_hi + 1) * S.mnz * SZ : 0);
 
     // sources of this tile, gathered into LDS (row-major TX x 64): one lane
     // per destination node, each summing its entries in the reference's order
+    // no sources: every row reads one zero slot (stride 0), so the row loop
+    // has no branch on the injection
+    T *zslot = zbuf + 4 * zrow_len<R>() - 1;  // the last parking slot of the last buffer, never written by rows
     c.has_inj = false;
-    c.finj = lds_f;
+    c.finj = zslot;
+    c.fstride = 0;
     if (B.inj_ptr) {
         const int n0 = B.inj_ptr[tile], n1 = B.inj_ptr[tile + 1];
         if (n1 > n0) {
@@ -420,18 +515,17 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
             }
             wave_sync();
             c.has_inj = true;
+            c.finj = lds_f + lane;
+            c.fstride = 64;
         }
     }
-
-    if (full) {
-        constexpr int NB = block_rows<T, MODE, true>();
-#pragma unroll 1
-        for (int jb = 0; jb < TX; jb += NB) rows_full<T, R, NB, MODE, PACKED>(c, P, x0 + jb, jb);
-    } else {
-        constexpr int NB = block_rows<T, MODE, false>();
-#pragma unroll 1
-        for (int jb = 0; jb < TX; jb += NB) rows_int<T, R, NB, MODE, PACKED>(c, P, x0 + jb, jb);
+    if (!c.has_inj) {
+        *zslot = T(0);
+        wave_sync();
     }
+
+    if (full) march_full<T, R, MODE, PACKED>(c, P, x0);
+    else march_int<T, R, MODE, PACKED>(c, P, x0);
 }
 
 // Receiver sampling: 16 lanes per receiver (4 receivers per wavefront); lane

@@ -25,6 +25,8 @@ namespace pa {
 constexpr int TY3 = 8;  // y rows per workgroup (one wavefront each)
 constexpr int TX3 = 8;  // x planes per tile
 
+int tile_rows_3d(int axis) { return axis == 0 ? TX3 : TY3; }
+
 template <typename T>
 struct Step3DParams {
     AxisCoef<T> cx, cy, cz;

@@ -28,7 +28,6 @@ import torch
 from . import _native as N
 
 TILE_Z = 64      # z nodes per wavefront (the contiguous axis)
-TILE_Y3 = 8      # y rows per 3D workgroup
 
 
 def z_tiles_2d(nz):
@@ -132,14 +131,12 @@ class DeviceBatch:
         self.deltas = tuple(float(d) for d in deltas)
         self.dt = float(dt)
         self.imaging_period = int(imaging_period)
-        if self.dim == 2:
-            # the 2D kernel's register blocking fixes the tile rows per dtype
-            need = 16 if self.dtype == np.float32 else 8
-            if tile_x and int(tile_x) != need:
-                raise ValueError(f"2D tiles are {need} rows along x for {self.dtype}")
-            self.tile_x = need
-        else:
-            self.tile_x = int(tile_x) if tile_x else 8
+        # the kernels' register blocking fixes the tile rows per dimension and dtype
+        need = N.tile_rows(self.dim, 1 if self.dtype == np.float64 else 0, 0)
+        if tile_x and int(tile_x) != need:
+            raise ValueError(f"{self.dim}D tiles are {need} rows along x for {self.dtype}")
+        self.tile_x = need
+        self.tile_y = N.tile_rows(self.dim, 1 if self.dtype == np.float64 else 0, 1) if self.dim == 3 else 1
         self.memory_budget = memory_budget
 
         if isinstance(model, torch.Tensor):
@@ -184,7 +181,7 @@ class DeviceBatch:
             return (ntx, 1, ntz), flags
         nx, ny, nz = s.shape
         ntz = -(-nz // TILE_Z)
-        nty = -(-ny // TILE_Y3)
+        nty = -(-ny // self.tile_y)
         ntx = -(-nx // self.tile_x)
         (xl, xr), (yl, yr), (zl, zr) = s.kpml
         flags = np.zeros(ntx * nty * ntz, dtype=np.int32)
@@ -192,7 +189,7 @@ class DeviceBatch:
             x0, x1 = tx * self.tile_x, min(nx, (tx + 1) * self.tile_x) - 1
             xp = (x0 - R < xl.size) or (x1 + R >= nx - xr.size)
             for ty in range(nty):
-                y0, y1 = ty * TILE_Y3, min(ny, (ty + 1) * TILE_Y3) - 1
+                y0, y1 = ty * self.tile_y, min(ny, (ty + 1) * self.tile_y) - 1
                 yp = (y0 - R < yl.size) or (y1 + R >= ny - yr.size)
                 for tz in range(ntz):
                     z0, z1 = tz * TILE_Z, min(nz, (tz + 1) * TILE_Z) - 1
@@ -210,8 +207,9 @@ class DeviceBatch:
             tz = np.where(z >= zlast, ntz - 1, z // TILE_Z)
             z0 = np.where(tz == ntz - 1, zlast, tz * TILE_Z)
             return (pos[:, 0] // self.tile_x) * ntz + tz, (pos[:, 0] % self.tile_x) * TILE_Z + (z - z0)
-        t = ((pos[:, 0] // self.tile_x) * nty + pos[:, 1] // TILE_Y3) * ntz + pos[:, 2] // TILE_Z
-        local = ((pos[:, 0] % self.tile_x) * TILE_Y3 + pos[:, 1] % TILE_Y3) * TILE_Z + pos[:, 2] % TILE_Z
+        ty = self.tile_y
+        t = ((pos[:, 0] // self.tile_x) * nty + pos[:, 1] // ty) * ntz + pos[:, 2] // TILE_Z
+        local = ((pos[:, 0] % self.tile_x) * ty + pos[:, 1] % ty) * TILE_Z + pos[:, 2] % TILE_Z
         return t, local
 
     def _build_tables(self):
@@ -407,7 +405,7 @@ class DeviceBatch:
         b.dim = self.dim
         b.order = self.order
         b.tile_x = self.tile_x
-        b.tile_y = TILE_Y3
+        b.tile_y = self.tile_y
         b.dt = self.dt
         b.dx = self.deltas[0]
         b.dy = self.deltas[1] if self.dim == 3 else 1.0
