@@ -55,7 +55,14 @@ def parse():
 
 
 def algorithmic_bytes(shapes_and_pml, itemsize):
-    """Per launch: forward and adjoint algorithmic bytes over all shots."""
+    """Algorithmic HBM bytes of one launch over all shots, the path's byte
+    model (s = itemsize, per padded node per time step):
+
+    forward: read u_k, u_{k-1} and the coefficient, write u_{k+1} (4s);
+    adjoint: the same (4s) plus the imaging condition, one stored forward
+    quantity read (s) and the FP64 accumulator read and written (16).  PML
+    nodes add Phix and Phiz read and written (4s) in either direction.
+    For FP32 this is 26 + 16 f_pml bytes per gradient grid-point update."""
     fwd = adj = 0.0
     for npts, npml in shapes_and_pml:
         base = 4 * itemsize * npts
@@ -204,14 +211,15 @@ def main():
                    "tile_x": problem.batch.tile_x},
         "roofline": {"bound": "hbm", "achieved": ach_adj, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach_adj / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_step2d<float,4,2> (adjoint + FP64 imaging)",
+                     "kernel": ("k_step2d<float,4,4> (adjoint + imaging, FP32 window sums into FP64)"
+                                if problem.batch.ic_window else "k_step2d<float,4,2> (adjoint + FP64 imaging)"),
                      "bytes_per_launch": b_adj, "avg_launch_us": adj_ms * 1e3, "launches": n_adj},
         "forward_kernel": {"kernel": "k_step2d<float,4,1> (forward + dWaveOp store)",
                            "achieved_gbs": ach_fwd, "frac": ach_fwd / HBM_PEAK_GBS,
                            "avg_launch_us": fwd_ms * 1e3, "bytes_per_launch": b_fwd, "launches": n_fwd},
         "gradient_roofline_frac": grad_bytes / t_step / 1e9 / HBM_PEAK_GBS,
         "bytes_per_update": grad_bytes / updates,
-        "misfit": float(loss), "grad_norm": gnorm,
+        "misfit": float(loss.detach()), "grad_norm": gnorm,
     }
 
     if not (args.no_plain or args.profile_run):

@@ -179,6 +179,13 @@ typedef struct {
      * same shape and window as the model); rhs_k = -m1 * dWaveOp0_k with
      * dWaveOp0 read from dw                                                 */
     const void *born_m1;
+    /* two-level imaging accumulator (FP32 2D batches, adjoint): when ic32 is
+     * not NULL the per-step products are summed into ic32 (FP32, total_n,
+     * zero on entry) and ic32 is added into ic (FP64) and cleared every
+     * ic_window slots and at the end of every pysit_amd_batch_run call, so ic
+     * holds the complete sum after each call.  NULL: every step adds into ic */
+    float *ic32;
+    int ic_window;
 } pysit_amd_batch;
 
 /* Run slots [slot_begin, slot_end) forward (direction = +1) or
@@ -186,7 +193,8 @@ typedef struct {
  *   mode 0: propagate only (+ receiver sampling when recs are given)
  *   mode 1: propagate and store dWaveOp for k % imaging_period == 0
  *   mode 2: adjoint: ic += q_k * dWaveOp_k (k % imaging_period == 0), then
- *           propagate; receiver sampling is skipped
+ *           propagate; receiver sampling is skipped (with ic32, the sum goes
+ *           through the FP32 window accumulator described above)
  *   mode 3: Born (linear_forward_model, temporal_modeling.py:389-546):
  *           propagate with the extra source -m1 * dWaveOp0_k read from dw
  *           (imaging_period must be 1), recording traces

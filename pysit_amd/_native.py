@@ -46,6 +46,7 @@ class Batch(ctypes.Structure):
         ("dw", _vp), ("ic", _vp),
         ("imaging_period", _i),
         ("fstore", _vp), ("born_m1", _vp),
+        ("ic32", _vp), ("ic_window", _i),
     ]
 
 

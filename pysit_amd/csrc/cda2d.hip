@@ -40,6 +40,8 @@
 // No MFMA: there is no dense contraction; the step is bound by HBM bandwidth.
 #include "common.h"
 
+#include <type_traits>
+
 // No implicit FP contraction: every fused multiply-add below is written out,
 // so all kernel modes (propagate, store dWaveOp, adjoint, Born) evaluate the
 // step with the same roundings and a checkpoint recompute reproduces the
@@ -180,6 +182,29 @@ __device__ __forceinline__ void z_taps(const Ctx2D<T, R, PACKED> &c, int buf, T 
     }
 }
 
+// Kernel modes: 0 propagate, 1 propagate + store dWaveOp, 2 adjoint with the
+// FP64 imaging accumulator, 3 Born, 4 adjoint with the FP32 window
+// accumulator (flushed into the FP64 one by the host loop).
+constexpr bool is_adjoint(int mode) { return mode == 2 || mode == 4; }
+
+template <typename T, int MODE> struct Acc2D { typedef double type; };
+template <typename T> struct Acc2D<T, 4> { typedef float type; };
+
+// imaging accumulator of the owned node (per-lane offset `o`, kOOB if masked)
+// in row byte offset xr of the T-typed fields
+template <typename T, int MODE, typename C>
+__device__ __forceinline__ typename Acc2D<T, MODE>::type ic_load(const C &c, int o, int xr)
+{
+    if constexpr (MODE == 4) return buf_ld<float>(c.ic, o + xr, c.shb);
+    else return buf_ld<double>(c.ic, o + xr * (8 / (int)sizeof(T)), c.shb8);
+}
+template <typename T, int MODE, typename C>
+__device__ __forceinline__ void ic_store(const C &c, int o, int xr, typename Acc2D<T, MODE>::type acc, T q, T dw)
+{
+    if constexpr (MODE == 4) buf_st(c.ic, o + xr, c.shb, acc + q * dw);
+    else buf_st(c.ic, o + xr * (8 / (int)sizeof(T)), c.shb8, acc + (double)q * (double)dw);
+}
+
 // Rows of a tile are computed in order (a march along x).  Each u row is
 // loaded once per tile; the loads of the rows `PA2D_PREFETCH` ahead are
 // issued before the current row is computed, so every wave keeps a steady
@@ -201,7 +226,7 @@ __device__ __forceinline__ void march_int(const Ctx2D<T, R, PACKED> &c, const St
     const int rb = c.rowb;
     const int so_u = c.s.so(F_U), so_uo = c.s.so(F_UO);
     T u[NU], uh[TX], um[TX], cv[TX], fv[TX], dv[TX], mv[TX];
-    double icv[TX];
+    typename Acc2D<T, MODE>::type icv[TX];
     const int zv = c.zv, hv = c.hv, ov = c.ov, ovd = c.ovd, ovi = c.ovi, ovf = c.ovf, mvo = c.mv;
 #define PA_LD_U(s_) u[s_] = buf_ld<T>(c.s.rs(F_U), zv + (x0 - R + (s_)) * rb, so_u)
 #define PA_LD_ROW(r_)                                                                               \
@@ -214,7 +239,7 @@ __device__ __forceinline__ void march_int(const Ctx2D<T, R, PACKED> &c, const St
         fv[r] = c.finj[r * c.fstride];                                                              \
         dv[r] = (MODE >= 2) ? buf_ld<T>(c.dw, ovd + xr, c.shb) : T(0);                              \
         mv[r] = (MODE == 3) ? buf_ld<T>(c.m1, mvo, mr) : T(0);                                      \
-        icv[r] = (MODE == 2) ? buf_ld<double>(c.ic, ovi + xr * (8 / (int)sizeof(T)), c.shb8) : 0.0; \
+        if (is_adjoint(MODE)) icv[r] = ic_load<T, MODE>(c, ovi, xr);                                \
     } while (0)
 #pragma unroll
     for (int q = 0; q < 2 * R + D && q < NU; ++q) PA_LD_U(q);
@@ -245,7 +270,7 @@ __device__ __forceinline__ void march_int(const Ctx2D<T, R, PACKED> &c, const St
         const T c2 = cv[j] * cv[j];
         const T src = f + lap;
         const T up = fma(P.tc.dt2, c2 * src, T(2) * v - um[j]);
-        if (MODE == 2) buf_st(c.ic, ovi + xr * (8 / (int)sizeof(T)), c.shb8, icv[j] + (double)v * (double)dv[j]);
+        if (is_adjoint(MODE)) ic_store<T, MODE>(c, ovi, xr, icv[j], v, dv[j]);
         buf_st(c.s.rs(F_UO), ov + xr, so_uo, up);
         if (MODE == 1) buf_st(c.dw, ovd + xr, c.shb, c2 * src);
         buf_st(c.fst, ovf + xr, c.shb, v);
@@ -267,7 +292,7 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
     // addresses and keep the context in memory
     const int zv = c.zv, zpv = c.zpv, hv = c.hv, hpv = c.hpv, ov = c.ov, ovd = c.ovd, ovi = c.ovi, ovf = c.ovf;
     T u[NU], px[NU], uh[TX], um[TX], cv[TX], fv[TX], dv[TX], mv[TX], pv[TX], ph[TX];
-    double icv[TX];
+    typename Acc2D<T, MODE>::type icv[TX];
     const int mvo = c.mv;
 #define PA_LD_U(s_)                                                                                 \
     do {                                                                                            \
@@ -295,7 +320,7 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
         fv[r] = f;                                                                                  \
         dv[r] = (MODE >= 2) ? buf_ld<T>(c.dw, own ? ovd + xr : kOOB, c.shb) : T(0);                 \
         mv[r] = (MODE == 3) ? buf_ld<T>(c.m1, mvo, mr) : T(0);                                      \
-        icv[r] = (MODE == 2) ? buf_ld<double>(c.ic, own ? ovi + xr * (8 / (int)sizeof(T)) : kOOB, c.shb8) : 0.0; \
+        if (is_adjoint(MODE)) icv[r] = ic_load<T, MODE>(c, own ? ovi : kOOB, xr);                  \
         pv[r] = buf_ld<T>(c.s.rs(F_PZ), own ? (xp ? zv : zpv) + xr : kOOB, so_pz);                  \
         ph[r] = buf_ld<T>(c.s.rs(F_PZ), own ? (xp ? hv : hpv) + xr : kOOB, so_pz);                  \
     } while (0)
@@ -371,8 +396,7 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
             dw = T(0);
             phi_store = true;
         }
-        if (MODE == 2)
-            buf_st(c.ic, own ? ovi + xr * (8 / (int)sizeof(T)) : kOOB, c.shb8, icv[j] + (double)v * (double)dv[j]);
+        if (is_adjoint(MODE)) ic_store<T, MODE>(c, own ? ovi : kOOB, xr, icv[j], v, dv[j]);
         buf_st(c.s.rs(F_UO), ovr, so_uo, up);
         const int op = phi_store ? ovr : kOOB;
         buf_st(c.s.rs(F_PXO), op, c.s.so(F_PXO), pxn);
@@ -422,7 +446,7 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
     c.hpv = hzp ? c.hv : kOOB;
     c.zedge = z == 0 || z >= nz - 1;
     // which buffer holds time level k: parity of the steps this propagation took
-    const int taken = (MODE == 2) ? (S.nt - 1 - k) : k;
+    const int taken = is_adjoint(MODE) ? (S.nt - 1 - k) : k;
     const int cur = taken & 1;
     // disabled arrays get an empty resource on a valid base: every access is
     // out of range, loads return 0 and stores are dropped
@@ -478,7 +502,7 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
         const int xi = (full && lane < 2 * TX) ? (lane < TX ? max(0, min(xs, nx - 2)) : xs) : -1;
         sigma_lane<T>(rxl, S.nxl, rxr, S.nxr, xi, nx, lane < TX, c.vsig2);
     }
-    c.img = (MODE == 2) && (k % B.imaging_period == 0) && B.dw;
+    c.img = is_adjoint(MODE) && (k % B.imaging_period == 0) && B.dw;
     const bool store = (MODE == 1) && (k % B.imaging_period == 0) && B.dw;
     const long long slot_el = (long long)(slot - seg_base) * B.total_n;
     const long long slot_bytes = B.total_n * SZ;
@@ -487,9 +511,10 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
     c.has_dwo = store;
     c.has_fst = B.fstore != nullptr;
     c.fst = make_rsrc(c.has_fst ? (const void *)((T *)B.fstore + slot_el) : vbase, c.has_fst ? slot_bytes : 0);
-    c.ic = make_rsrc(MODE == 2 ? (const void *)B.ic : vbase, MODE == 2 ? B.total_n * 8 : 0);
+    if constexpr (MODE == 4) c.ic = make_rsrc(B.ic32, B.total_n * 4);
+    else c.ic = make_rsrc(MODE == 2 ? (const void *)B.ic : vbase, MODE == 2 ? B.total_n * 8 : 0);
     c.ovd = dwon ? c.ov : kOOB;
-    c.ovi = c.img ? c.ov8 : kOOB;
+    c.ovi = c.img ? (MODE == 4 ? c.ov : c.ov8) : kOOB;
     c.ovf = c.has_fst ? c.ov : kOOB;
     c.m1 = make_rsrc(MODE == 3 ? B.born_m1 : vbase, MODE == 3 ? (long long)(S.mx_hi + 1) * S.mnz * SZ : 0);
 
@@ -563,7 +588,7 @@ __global__ __launch_bounds__(256) void k_step2d(const pysit_amd_batch B, const S
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int b = blockIdx.x;
-    if (MODE != 2 && b < rec_blocks) {
+    if (!is_adjoint(MODE) && b < rec_blocks) {
         const int rw = b * 4 + w;
         const int rec = rw * 4 + (lane >> 4);
         if (rw * 4 < n_rec_lanes) sample_group<T>(B, slot, rec < n_rec_lanes ? rec : B.n_rec, lane & 15);
@@ -583,7 +608,7 @@ template <typename T, int R, int MODE>
 static int launch2d(const pysit_amd_batch &B, const Step2DParams<T> &P, int slot, int seg_base,
                     int n_tiles, int n_recs, hipStream_t st)
 {
-    const int recs = (MODE != 2) ? n_recs : 0;
+    const int recs = is_adjoint(MODE) ? 0 : n_recs;
     // 16 receivers per block, block count rounded to whole XCD rounds
     const int rec_blocks = recs ? ((recs + 15) / 16 + 7) / 8 * 8 : 0;
     const int tile_blocks = (n_tiles + 3) / 4;
@@ -620,6 +645,24 @@ static int launch2d_r(int r, const pysit_amd_batch &B, const Step2DParams<T> &P,
 
 int tile_rows_2d(int dtype) { return dtype ? Tile2D<double>::TX : Tile2D<float>::TX; }
 
+// ic += ic32; ic32 = 0 (the FP32 window accumulator's flush)
+__global__ void k_ic_flush(float *__restrict__ ic32, double *__restrict__ ic, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ic[i] += (double)ic32[i];
+    ic32[i] = 0.0f;
+}
+
+static int ic_flush(const pysit_amd_batch &B, hipStream_t st)
+{
+    const long long n = B.total_n;
+    if (n == 0) return PYSIT_AMD_OK;
+    hipLaunchKernelGGL(k_ic_flush, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, B.ic32, B.ic, n);
+    PA_HIP(hipGetLastError());
+    return PYSIT_AMD_OK;
+}
+
 template <typename T>
 int run2d(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int direction,
           int seg_base, const int *active_tiles, const int *active_recs, hipStream_t st)
@@ -655,6 +698,7 @@ int run2d(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int 
         P.state_bytes = (int)ext;
     }
     PA_CHECK(B.total_n * 8 <= kMaxRsrcBytes, "batch too large for one launch (total_n)");
+    const bool window = mode == 2 && B.ic32 != nullptr;
     const int n = slot_end - slot_begin;
     for (int i = 0; i < n; ++i) {
         const int j = direction > 0 ? slot_begin + i : slot_end - 1 - i;
@@ -663,8 +707,13 @@ int run2d(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int 
         int rc;
         if (mode == 0) rc = launch2d_r<T, 0>(r, B, P, j, seg_base, nt, nr, st);
         else if (mode == 1) rc = launch2d_r<T, 1>(r, B, P, j, seg_base, nt, nr, st);
-        else if (mode == 2) rc = launch2d_r<T, 2>(r, B, P, j, seg_base, nt, 0, st);
-        else rc = launch2d_r<T, 3>(r, B, P, j, seg_base, nt, nr, st);
+        else if (mode == 3) rc = launch2d_r<T, 3>(r, B, P, j, seg_base, nt, nr, st);
+        else if (!window) rc = launch2d_r<T, 2>(r, B, P, j, seg_base, nt, 0, st);
+        else {
+            if constexpr (std::is_same<T, float>::value) rc = launch2d_r<T, 4>(r, B, P, j, seg_base, nt, 0, st);
+            else rc = PYSIT_AMD_EINVAL;
+            if (!rc && ((i + 1) % B.ic_window == 0 || i + 1 == n)) rc = ic_flush(B, st);
+        }
         if (rc) return rc;
     }
     return PYSIT_AMD_OK;

@@ -113,7 +113,7 @@ class DeviceBatch:
     """Tables and state of a shot batch on one GPU."""
 
     def __init__(self, shots, model, deltas, dt, order, dtype=np.float32, tile_x=None,
-                 imaging_period=1, memory_budget=None):
+                 imaging_period=1, memory_budget=None, ic_window=None):
         if not shots:
             raise ValueError("empty shot batch")
         self.dev = _dev()
@@ -138,6 +138,13 @@ class DeviceBatch:
         self.tile_x = need
         self.tile_y = N.tile_rows(self.dim, 1 if self.dtype == np.float64 else 0, 1) if self.dim == 3 else 1
         self.memory_budget = memory_budget
+        # FP32 2D adjoints sum the imaging condition in FP32 over windows of
+        # `ic_window` slots and in FP64 across windows (0: FP64 every step)
+        if ic_window is None:
+            ic_window = 32 if (self.dim == 2 and self.dtype == np.float32) else 0
+        if ic_window and not (self.dim == 2 and self.dtype == np.float32):
+            raise ValueError("the FP32 window accumulator is for 2D FP32 batches")
+        self.ic_window = int(ic_window)
 
         if isinstance(model, torch.Tensor):
             self.model = model.detach().to(device=self.dev, dtype=self.tdtype).contiguous()
@@ -605,10 +612,15 @@ class DeviceBatch:
         for f in self.qstate.values():
             f.zero_()
         self.ic = torch.zeros(self.total_n, dtype=torch.float64, device=self.dev)
+        ic32 = torch.zeros(self.total_n, dtype=torch.float32, device=self.dev) if self.ic_window else None
         K = getattr(self, "K", self.max_nt)
         if not hasattr(self, "ckpts"):
             self.ckpts = []
         qb = self._batch(self.qstate, True, ic=self.ic, dw=None)
+        if ic32 is not None:
+            qb.ic32 = ic32.data_ptr()
+            qb.ic_window = self.ic_window
+            self._ic32 = ic32
         self.adjoint_store = None
         if store_field:
             self.adjoint_store = self._field_store()

@@ -56,13 +56,14 @@ class TemporalModeling(object):
     solver_type = "time"
     modeling_type = "time"
 
-    def __init__(self, solver, memory_budget=None, tile_x=None):
+    def __init__(self, solver, memory_budget=None, tile_x=None, ic_window=None):
         if self.solver_type != solver.supports["equation_dynamics"]:
             raise TypeError(f"Argument 'solver' type {solver.supports['equation_dynamics']} does not match "
                             f"modeling solver type {self.solver_type}.")
         self.solver = solver
         self.memory_budget = memory_budget
         self.tile_x = tile_x
+        self.ic_window = ic_window  # None: the batch default (FP32 2D: 32 slots)
 
     # ---------------------------------------------------------------- plumbing
     def geometry(self, shot):
@@ -79,7 +80,7 @@ class TemporalModeling(object):
         s = self.solver
         return DeviceBatch([self.geometry(sh) for sh in shots], s.unpadded_model(), s.mesh.deltas, s.dt,
                            s.spatial_accuracy_order, dtype=s.dtype, imaging_period=imaging_period,
-                           memory_budget=self.memory_budget, tile_x=self.tile_x)
+                           memory_budget=self.memory_budget, tile_x=self.tile_x, ic_window=self.ic_window)
 
     def _unpad_field(self, padded):
         mesh = self.solver.mesh

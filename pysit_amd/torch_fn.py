@@ -29,11 +29,12 @@ class ShotProblem(object):
     """Shots of one solver / mesh for the autograd path (geometry built once
     per model change by ``TemporalModeling.geometry``)."""
 
-    def __init__(self, solver, shots, imaging_period=1, memory_budget=None, tile_x=None, segment=None):
+    def __init__(self, solver, shots, imaging_period=1, memory_budget=None, tile_x=None, segment=None,
+                 ic_window=None):
         from .modeling.temporal_modeling import TemporalModeling
         self.solver = solver
         self.shots = shots
-        self.tm = TemporalModeling(solver, memory_budget=memory_budget, tile_x=tile_x)
+        self.tm = TemporalModeling(solver, memory_budget=memory_budget, tile_x=tile_x, ic_window=ic_window)
         self.imaging_period = imaging_period
         self.segment = segment
         self.batch = None
@@ -56,7 +57,8 @@ class ShotProblem(object):
             self.batch = None
             self.batch = DeviceBatch([self.tm.geometry(sh) for sh in self.shots], C, s.mesh.deltas, s.dt,
                                      s.spatial_accuracy_order, dtype=s.dtype, imaging_period=self.imaging_period,
-                                     memory_budget=self.tm.memory_budget, tile_x=self.tm.tile_x)
+                                     memory_budget=self.tm.memory_budget, tile_x=self.tm.tile_x,
+                                     ic_window=self.tm.ic_window)
             self._key = key
         else:
             self.batch.model.copy_(C.reshape(self.batch.model.shape))

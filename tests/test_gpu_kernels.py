@@ -170,6 +170,24 @@ def test_batch_fp32_close_to_fp64_oracle(gpu):
     assert rel_l2(img, img_ref) < 1e-3
 
 
+def test_ic_window_accumulator_matches_fp64_accumulation(gpu):
+    # FP32 batch: imaging sums in FP32 windows of 7 slots (400 steps: the last
+    # window is partial), flushed into FP64, against FP64 accumulation every
+    # step; checkpointed segments of 50 slots also flush at every segment end
+    p = Synthetic((90, 110), ((10, 10), (10, 10)), (0.01, 0.01), 400, 8, seed=4)
+    rng = np.random.default_rng(7)
+    res = [rng.standard_normal((400, p.n_rec))]
+    imgs = []
+    for w, seg in ((0, None), (7, None), (7, 50), (32, None)):
+        b = _batch([p], np.float32, ic_window=w)
+        b.forward(keep_dw=True, segment=seg)
+        b.migrate(res)
+        imgs.append(b.image(0).cpu().numpy().reshape(-1).copy())
+    ref = imgs[0]
+    for img in imgs[1:]:
+        assert rel_l2(img, ref) < 1e-5
+
+
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-10), (np.float32, 1e-3)])
 def test_batch_3d(gpu, dtype, tol):
     p = Synthetic((20, 22, 70), ((4, 4), (3, 5), (4, 3)), (0.01, 0.011, 0.012), 90, 8, seed=5, n_rec=5)
