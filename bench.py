@@ -51,6 +51,9 @@ def parse():
     p.add_argument("--tile-x", type=int, default=None)
     p.add_argument("--memory-budget-gb", type=float, default=None)
     p.add_argument("--profile-run", action="store_true", help="minimal run for rocprofv3 (no plain, no CPU)")
+    p.add_argument("--pmc-run", action="store_true",
+                   help="short run for rocprofv3 --pmc: a 256 MiB elementwise calibration kernel, then one "
+                        "gradient of the workload over --nt slots (default 40)")
     return p.parse_args()
 
 
@@ -118,6 +121,29 @@ def cpu_baseline(shape, pml_pairs, dt, deltas, seconds):
                        f"from above (imaging and dWaveOp work not counted)")}
 
 
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "traffic_k_step2d_layered2d.json")
+
+
+def measured_traffic(shots, shape):
+    """HBM bytes per launch of the forward and adjoint kernels from the
+    committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE per
+    dispatch, each scaled by the factor the same counter needs on a copy of
+    known size), when it was measured on this workload; else nulls."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return {}
+    if t.get("shots") != shots or list(t.get("grid_padded", [])) != list(shape):
+        return {}
+    out = {"source": os.path.relpath(TRAFFIC_FILE, os.path.dirname(TRAFFIC_FILE) + "/..")}
+    for k in ("forward", "adjoint"):
+        d = t["kernels"][k]
+        out[k] = d["fetch_kib"] * 1024 * t["fetch_factor"] + d["write_kib"] * 1024 * t["write_factor"]
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -128,6 +154,16 @@ def main():
     import pysit_amd as pa
     from pysit_amd.torch_fn import ShotProblem, ShotTraces
     from pysit_amd.workloads import layered_2d
+    if args.pmc_run:
+        # calibration: y = x + 1 over 2^26 floats reads and writes exactly 256 MiB
+        x = torch.ones(2 ** 26, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.add(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        del x, y
+        if args.nt == 3000:
+            args.nt = 40
 
     dev = torch.device("cuda", 0)
     mesh, shots, solver, model = layered_2d(n_shots=args.shots, nt=args.nt)
@@ -184,6 +220,7 @@ def main():
 
     fwd_ms, n_fwd = avg_ms(1)
     adj_ms, n_adj = avg_ms(2)
+    traffic = measured_traffic(len(shots), shape)
     itemsize = 4 if solver.dtype == np.single else 8
     npml = pml_nodes(shape, pads)
     b_fwd, b_adj = algorithmic_bytes([(npts, npml)] * len(shots), itemsize)
@@ -210,18 +247,23 @@ def main():
                    "grid_padded": list(shape), "shots": len(shots), "nt": nt, "order": 8,
                    "tile_x": problem.batch.tile_x},
         "roofline": {"bound": "hbm", "achieved": ach_adj, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach_adj / HBM_PEAK_GBS, "traffic": None,
+                     "frac": ach_adj / HBM_PEAK_GBS, "traffic": traffic.get("adjoint"),
                      "kernel": ("k_step2d<float,4,4> (adjoint + imaging, FP32 window sums into FP64)"
                                 if problem.batch.ic_window else "k_step2d<float,4,2> (adjoint + FP64 imaging)"),
                      "bytes_per_launch": b_adj, "avg_launch_us": adj_ms * 1e3, "launches": n_adj},
         "forward_kernel": {"kernel": "k_step2d<float,4,1> (forward + dWaveOp store)",
                            "achieved_gbs": ach_fwd, "frac": ach_fwd / HBM_PEAK_GBS,
-                           "avg_launch_us": fwd_ms * 1e3, "bytes_per_launch": b_fwd, "launches": n_fwd},
+                           "avg_launch_us": fwd_ms * 1e3, "bytes_per_launch": b_fwd, "launches": n_fwd,
+                           "traffic": traffic.get("forward")},
+        "traffic_source": traffic.get("source"),
         "gradient_roofline_frac": grad_bytes / t_step / 1e9 / HBM_PEAK_GBS,
         "bytes_per_update": grad_bytes / updates,
         "misfit": float(loss.detach()), "grad_norm": gnorm,
     }
 
+    if args.pmc_run:
+        print(json.dumps({"pmc_run": "done", "nt": nt, "shots": len(shots)}), flush=True)
+        return
     if not (args.no_plain or args.profile_run):
         plain = [ShotProblem(solver, [s], memory_budget=budget, tile_x=args.tile_x) for s in shots]
         for p in plain[:1]:

@@ -273,7 +273,7 @@ __device__ __forceinline__ void march_int(const Ctx2D<T, R, PACKED> &c, const St
         if (is_adjoint(MODE)) ic_store<T, MODE>(c, ovi, xr, icv[j], v, dv[j]);
         buf_st(c.s.rs(F_UO), ov + xr, so_uo, up);
         if (MODE == 1) buf_st(c.dw, ovd + xr, c.shb, c2 * src);
-        buf_st(c.fst, ovf + xr, c.shb, v);
+        if (c.has_fst) buf_st(c.fst, ovf + xr, c.shb, v);
     }
 #undef PA_LD_U
 #undef PA_LD_ROW
@@ -315,7 +315,7 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
         um[r] = buf_ld<T>(c.s.rs(F_UO), ovr, so_uo);                                                \
         const int mr = clampi(xs - c.mx_off, c.mx_lo, c.mx_hi) * c.mrowb;                          \
         cv[r] = buf_ld<T>(c.model, mvo, mr);                                                        \
-        T f = buf_ld<T>(c.rhs, ovr, 0); /* zero-sized resource without a dense rhs */               \
+        T f = PACKED ? T(0) : buf_ld<T>(c.rhs, ovr, 0); /* dense rhs: reference-contract step */   \
         f += c.finj[r * c.fstride];                                                                 \
         fv[r] = f;                                                                                  \
         dv[r] = (MODE >= 2) ? buf_ld<T>(c.dw, own ? ovd + xr : kOOB, c.shb) : T(0);                 \
@@ -402,7 +402,7 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
         buf_st(c.s.rs(F_PXO), op, c.s.so(F_PXO), pxn);
         buf_st(c.s.rs(F_PZO), op, c.s.so(F_PZO), pzn);
         if (MODE == 1) buf_st(c.dw, own ? ovd + xr : kOOB, c.shb, dw);
-        buf_st(c.fst, own ? ovf + xr : kOOB, c.shb, v);
+        if (c.has_fst) buf_st(c.fst, own ? ovf + xr : kOOB, c.shb, v);
     }
 #undef PA_LD_U
 #undef PA_LD_ROW
@@ -578,8 +578,11 @@ __device__ __forceinline__ void sample_group(const pysit_amd_batch &B, int slot,
 // Grid: the first `rec_blocks` blocks sample receivers (dispatched first, so
 // their short gather chains overlap the tiles), the rest own tiles.  SINGLE
 // is the reference-contract step: one shot passed by value, every tile PML.
+#ifndef PA2D_WAVES_PER_EU
+#define PA2D_WAVES_PER_EU 1
+#endif
 template <typename T, int R, int MODE, bool SINGLE>
-__global__ __launch_bounds__(256) void k_step2d(const pysit_amd_batch B, const Step2DParams<T> P,
+__global__ __launch_bounds__(256, PA2D_WAVES_PER_EU) void k_step2d(const pysit_amd_batch B, const Step2DParams<T> P,
                                                 int slot, int seg_base, int n_tile_waves,
                                                 int n_rec_lanes, int rec_blocks)
 {

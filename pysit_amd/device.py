@@ -113,7 +113,7 @@ class DeviceBatch:
     """Tables and state of a shot batch on one GPU."""
 
     def __init__(self, shots, model, deltas, dt, order, dtype=np.float32, tile_x=None,
-                 imaging_period=1, memory_budget=None, ic_window=None):
+                 imaging_period=1, memory_budget=None, ic_window=None, tile_order="shot"):
         if not shots:
             raise ValueError("empty shot batch")
         self.dev = _dev()
@@ -145,6 +145,12 @@ class DeviceBatch:
         if ic_window and not (self.dim == 2 and self.dtype == np.float32):
             raise ValueError("the FP32 window accumulator is for 2D FP32 batches")
         self.ic_window = int(ic_window)
+        # tile table order: "shot" (all tiles of a shot together: halo rows of
+        # neighbouring tiles are reused in L2) or "interleave" (the shots of
+        # one tile together: the model window is reused; alike shots only)
+        if tile_order not in ("shot", "interleave"):
+            raise ValueError("tile_order must be 'shot' or 'interleave'")
+        self.tile_order = tile_order
 
         if isinstance(model, torch.Tensor):
             self.model = model.detach().to(device=self.dev, dtype=self.tdtype).contiguous()
@@ -235,7 +241,7 @@ class DeviceBatch:
         # tile table: interleave the shots of one tile when they are alike,
         # else shot-major (active prefix by kshift)
         same = all(s.shape == self.shots[0].shape and s.window == self.shots[0].window
-                   for s in self.shots) and len(set(self.kshift)) == 1
+                   for s in self.shots) and len(set(self.kshift)) == 1 and self.tile_order == "interleave"
         rows = []
         if same:
             ntile = len(tiles_info[0][1])
