@@ -10,6 +10,8 @@ package's own API, seeded so that every run sees the same numbers:
   ``n_layers`` dipping layers (seeded dips), times (1 + 0.05 g) with g a
   Gaussian random field smoothed with sigma = 150 m and scaled to unit RMS,
   clipped to [vmin, vmax]; PML of ``pml_cells`` nodes.
+* ``uneven_2d`` -- shots of seeded random aperture (a window of the layered
+  model's columns) and step count, as device geometries for one batch.
 * ``layered_3d`` -- the 3D analogue with a Gaussian low-velocity anomaly.
 """
 import numpy as np
@@ -19,8 +21,8 @@ from .core import (RectangularDomain, PML, Dirichlet, CartesianMesh, RickerWavel
                    ReceiverSet, Shot, equispaced_acquisition)
 from .solvers import ConstantDensityAcousticWave
 
-__all__ = ["horizontal_reflector_2d", "layered_velocity_2d", "layered_2d", "layered_velocity_3d", "layered_3d",
-           "pml_amplitude", "trange_for_steps"]
+__all__ = ["horizontal_reflector_2d", "layered_velocity_2d", "layered_2d", "uneven_2d", "layered_velocity_3d",
+           "layered_3d", "pml_amplitude", "trange_for_steps"]
 
 
 def pml_amplitude(vmax, width, reflection=1e-3):
@@ -89,6 +91,49 @@ def layered_2d(nx=1001, nz=301, dx=10.0, n_shots=16, nt=3000, order=8, precision
     solver.model_parameters = model
     assert solver.nsteps == nt, (solver.nsteps, nt)
     return m, shots, solver, model
+
+
+def uneven_2d(n_shots=64, nx=1001, nz=301, dx=10.0, order=8, pml_cells=20, f0=8.0, depth=20.0,
+              aperture=(300, 1001), nt_range=(1500, 4000), seed=0, model=None):
+    """Shots of uneven aperture and recording length over the layered model.
+
+    Shot s sees columns [a, a + w) of the nx x nz model, w and the step count
+    drawn uniformly from ``aperture`` and ``nt_range`` (seeded); its padded
+    grid is (w + 2 pml_cells) x (nz + 2 pml_cells), edge-padded from that
+    window.  One Ricker source at the aperture centre, one receiver per
+    aperture column, both at ``depth``.  All shots share the CFL dt of the
+    whole model so they run in one batch.
+
+    Returns (geometries, model (nx, nz), deltas, dt, per-shot metadata)."""
+    from .core.receivers import sampling_entries, adjoint_entries
+    from .device import ShotGeometry
+    v = layered_velocity_2d(nx, nz, dx, seed=seed) if model is None else np.asarray(model, dtype=np.float64)
+    rng = np.random.default_rng(seed + 1)
+    width = pml_cells * dx
+    A = pml_amplitude(v.max(), width)
+    dt = (1.0 / 6.0) * dx / v.max()
+    wav = RickerWavelet(f0)
+    geos, meta = [], []
+    for _ in range(n_shots):
+        w = int(rng.integers(aperture[0], aperture[1] + 1))
+        a = int(rng.integers(0, nx - w + 1))
+        nt = int(rng.integers(nt_range[0], nt_range[1] + 1))
+        p = PML(width, A)
+        d = RectangularDomain((a * dx, (a + w - 1) * dx, p, p), (0.0, (nz - 1) * dx, p, p))
+        m = CartesianMesh(d, w, nz)
+        src = PointSource(m, ((a + 0.5 * (w - 1)) * dx, depth), wav)
+        recs = ReceiverSet(m, [PointReceiver(m, ((a + i) * dx, depth)) for i in range(w)])
+        shape = tuple(m.shape(include_bc=True, as_grid=True))
+        inj, amp = src.injection(dt, nt)
+        kpml = []
+        for prm in (m.parameters[0], m.parameters[1]):
+            kpml.append((prm.lbc.sigma if prm.lbc.type == "pml" else np.zeros(0),
+                         prm.rbc.sigma if prm.rbc.type == "pml" else np.zeros(0)))
+        window = [(pml_cells - a, a, a + w - 1), (pml_cells, 0, nz - 1)]
+        geos.append(ShotGeometry(shape, kpml, window, nt, inj, amp, sampling_entries(recs, shape), w,
+                                 adj=adjoint_entries(recs)))
+        meta.append({"a": a, "w": w, "nt": nt, "shape": shape})
+    return geos, v, (dx, dx), dt, meta
 
 
 def layered_velocity_3d(n, dx, vmin=1500.0, vmax=4000.0, n_layers=8, seed=0, anomaly_radius=400.0,

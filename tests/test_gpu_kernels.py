@@ -170,6 +170,26 @@ def test_batch_fp32_close_to_fp64_oracle(gpu):
     assert rel_l2(img, img_ref) < 1e-3
 
 
+def test_fp32_accuracy_over_a_long_run(gpu):
+    # 2000 steps in FP32 against the FP64 oracle: the error grows with the
+    # step count like the reference's own FP32 arithmetic does (about 1.9e-4
+    # on a 3000-step shot of the 1041 x 341 layered case).  Here it measures
+    # 1.5e-5 (traces) and 9e-6 (gradient).
+    p = Synthetic((90, 110), ((10, 10), (10, 10)), (0.01, 0.01), 2000, 8, seed=4)
+    b = _batch([p], np.float32)
+    tr = b.forward(keep_dw=True)[0].cpu().numpy()
+    op = p.oracle_problem()
+    ref = op.forward()["simdata"]
+    rng = np.random.default_rng(2)
+    res = [0.1 * ref + 0.01 * rng.standard_normal(ref.shape) * np.abs(ref).max()]
+    b.migrate(res)
+    img = b.image(0).cpu().numpy().reshape(-1) * (-p.dt)
+    img_ref = op.migrate(res[0], fold=False)
+    e_tr, e_img = rel_l2(tr, ref), rel_l2(img, img_ref)
+    print(f"FP32, 2000 steps: traces {e_tr:.2e}, gradient {e_img:.2e}")
+    assert e_tr < 3e-5 and e_img < 3e-5, (e_tr, e_img)
+
+
 def test_ic_window_accumulator_matches_fp64_accumulation(gpu):
     # FP32 batch: imaging sums in FP32 windows of 7 slots (400 steps: the last
     # window is partial), flushed into FP64, against FP64 accumulation every
