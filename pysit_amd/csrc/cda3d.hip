@@ -5,17 +5,28 @@
 // Phiz, psi; taps beyond the grid read 0; edge nodes held at 0; psi += dt*u
 // on every updated node; right-side PML index quirk as in 2D).
 //
-// Mapping to CDNA: a 512-thread workgroup owns a 64 (z, contiguous) x 8 (y)
-// column of the grid and walks `tile_x` planes along x.  Each thread keeps
-// register queues along x (u, and in PML tiles Phix and psi); the current
-// y-z plane of u (and of psi / Phiy / Phiz in PML tiles) is staged in LDS
-// with an r-node halo for the y and z derivatives.  Tiles farther than r
-// nodes from every PML node never touch Phi or psi: nothing there can read
-// them (Phi is zero off the PML, and psi only enters through sigma-weighted
-// terms and PML-node derivatives), so they are neither read nor written.
-// The step entry point with the reference contract marks every tile as PML
-// so that it returns all five fields exactly as the reference defines them.
+// Mapping to CDNA: a 512-thread workgroup owns 64 z (contiguous) x 8 y x 8 x
+// nodes; wavefront w owns row y0 + w.  The z tiles of a shot start at 0, 64,
+// ... except the last, which ends on the last node (owned ranges disjoint).
+// The workgroup marches through its 8 x planes: each lane keeps a register
+// queue of its own row along x (u, and in PML tiles Phix and psi) for the x
+// derivatives, and the current y-z plane is staged in a double-buffered LDS
+// plane for the y and z derivatives (one barrier per plane).  Every wave
+// loads its own row, the z halo of its own row (lanes 0..r-1 and 64-r..63),
+// and waves 0..2r-1 one y-halo row each.  The loads of the planes ahead are
+// issued before the current plane is staged, and the barrier waits on LDS
+// only, so they stay in flight.
+//
+// Addressing: per-field raw buffer resources based at the shot's region; a
+// per-lane byte offset (masked lanes: out of range, reads 0, stores dropped)
+// plus the plane offset.  Tiles farther than r nodes from every PML node,
+// edge node and the grid boundary run the plain update and never touch Phi or
+// psi (Phi is zero off the PML, psi only enters through sigma-weighted terms
+// and PML-node derivatives); the reference-contract step marks every tile as
+// PML so that it returns all five fields exactly as the reference defines them.
 #include "common.h"
+
+#include <type_traits>
 
 // No implicit FP contraction (see cda2d.hip): all modes round identically.
 #pragma clang fp contract(off)
@@ -27,6 +38,10 @@ constexpr int TX3 = 8;  // x planes per tile
 
 int tile_rows_3d(int axis) { return axis == 0 ? TX3 : TY3; }
 
+#ifndef PA3D_PREFETCH
+#define PA3D_PREFETCH 2
+#endif
+
 template <typename T>
 struct Step3DParams {
     AxisCoef<T> cx, cy, cz;
@@ -36,258 +51,439 @@ struct Step3DParams {
     pysit_amd_shot shot0;
 };
 
+// LDS of a workgroup: double-buffered planes (u and psi with y and z halos,
+// Phiy with y halo rows, Phiz with the z halo), the injection block, and a
+// write-off slot per lane for lanes that hold no halo value.
 template <typename T, int R>
-struct Planes3D {
-    T u[TY3 + 2 * R][64 + 2 * R];
-    T ps[TY3 + 2 * R][64 + 2 * R];
-    T py[TY3 + 2 * R][64];
-    T pz[TY3][64 + 2 * R];
+struct Lds3D {
+    static constexpr int LZ = 64 + 2 * R;
+    static constexpr int LY = TY3 + 2 * R;
+    T u[2][LY * LZ];
+    T ps[2][LY * LZ];
+    T py[2][LY * 64];
+    T pz[2][TY3 * LZ];
+    T f[TX3 * TY3 * 64];
+    T junk[4][64];
 };
+
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <typename T, int R>
+struct Ctx3D {
+    Rsrc u, uo, px, pxo, py, pyo, pz, pzo, ps, pso, model, m1, dw, fst, ic, rhs;
+    bool has_fst, has_rhs;
+    int nx, ny, nz, lane, w, y;
+    int plane;               // bytes per x plane of the shot grid
+    int co, cho, hro;        // own row centre, own row z-halo value, y-halo row centre (kOOB if none)
+    int ov, ov8, ovd, ovi, ovf;
+    int mv, mplane, mx_off, mx_lo, mx_hi;
+    int shb, shb8;
+    int hl;                  // LDS index (in a plane buffer) of this lane's z-halo slot in its own row, or -1
+    int hrow;                // LDS row of this wave's y-halo row, or -1
+    bool yzedge;             // y or z on the grid edge
+    T syk, sym, szk, szm;    // lane y/z profiles (kernel, mesh)
+    T vsig;                  // x profile of the tile planes: lane p < TX kernel, lane TX + p mesh
+    const T *finj;
+    int fstride;
+};
+
+// LDS store of this lane's z-halo value (into the write-off slot when the
+// lane holds none)
+template <typename T, int R>
+__device__ __forceinline__ void st_halo(T *plane, T *junk, int hl, int lane, T v)
+{
+    T *dst = hl >= 0 ? plane + hl : junk + lane;
+    *dst = v;
+}
+
+// PML profile value at node i (i < 0: none), kernel or mesh form, as two
+// masked buffer loads (see cda2d.hip)
+template <typename T>
+__device__ __forceinline__ T sigma3(Rsrc l, int nl, Rsrc r, int nr, int i, int n, bool kernel)
+{
+    constexpr int SZ = sizeof(T);
+    const bool inl = i >= 0 && i < nl;
+    const bool inr = !inl && i >= 0 && i >= n - nr && (!kernel || i <= n - 2);
+    const int ri = kernel ? nr - ((n - 1) - i) : i - (n - nr);
+    return buf_ld<T>(l, inl ? i * SZ : kOOB, 0) + buf_ld<T>(r, inr ? ri * SZ : kOOB, 0);
+}
+
+__device__ __forceinline__ float bcast3(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double bcast3(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// byte offset `o` plus plane offset `p`, kept out of range when o is
+__device__ __forceinline__ int at(int o, int p) { return o == kOOB ? kOOB : o + p; }
+
+// ---- plain tiles -------------------------------------------------------------
+template <typename T, int R, int MODE>
+__device__ __forceinline__ void march3_int(const Ctx3D<T, R> &c, const Step3DParams<T> &P, Lds3D<T, R> &L,
+                                           int x0)
+{
+    constexpr int TX = TX3, D = PA3D_PREFETCH, NQ = TX + 2 * R, LZ = Lds3D<T, R>::LZ;
+    constexpr int SZ = sizeof(T);
+    const int pl = c.plane;
+    const int co = c.co, cho = c.cho, hro = c.hro, ov = c.ov, mv = c.mv;
+    const int ovd = c.ovd, ovi = c.ovi, ovf = c.ovf;
+    T q[NQ], uz[TX], yh[TX], um[TX], cv[TX], fv[TX], dv[TX], mm[TX];
+    double icv[TX];
+#define PA_Q(s_) q[s_] = buf_ld<T>(c.u, at(co, (x0 - R + (s_)) * pl), 0)
+#define PA_ROW(r_)                                                                                  \
+    do {                                                                                            \
+        const int r = (r_), xp = (x0 + r) * pl;                                                     \
+        uz[r] = buf_ld<T>(c.u, at(cho, xp), 0);                                                     \
+        yh[r] = buf_ld<T>(c.u, at(hro, xp), 0);                                                     \
+        um[r] = buf_ld<T>(c.uo, at(ov, xp), 0);                                                     \
+        const int mp = clampi(x0 + r - c.mx_off, c.mx_lo, c.mx_hi) * c.mplane;                      \
+        cv[r] = buf_ld<T>(c.model, mv, mp);                                                         \
+        fv[r] = c.finj[r * c.fstride];                                                              \
+        dv[r] = (MODE >= 2) ? buf_ld<T>(c.dw, at(ovd, xp), c.shb) : T(0);                           \
+        mm[r] = (MODE == 3) ? buf_ld<T>(c.m1, mv, mp) : T(0);                                       \
+        icv[r] = (MODE == 2) ? buf_ld<double>(c.ic, at(ovi, xp * (8 / SZ)), c.shb8) : 0.0;          \
+    } while (0)
+#pragma unroll
+    for (int s = 0; s < 2 * R + D && s < NQ; ++s) PA_Q(s);
+#pragma unroll
+    for (int s = 0; s < D && s < TX; ++s) PA_ROW(s);
+    __builtin_amdgcn_sched_barrier(0);
+    const int own = (R + c.w) * LZ + R + c.lane;
+#pragma unroll
+    for (int j = 0; j < TX; ++j) {
+        if (j + 2 * R + D < NQ) PA_Q(j + 2 * R + D);
+        if (j + D < TX) PA_ROW(j + D);
+        __builtin_amdgcn_sched_barrier(0);
+        T *lu = L.u[j & 1];
+        const T v = q[R + j];
+        lu[own] = v;
+        st_halo<T, R>(lu, L.junk[0], c.hl, c.lane, uz[j]);
+        if (c.hrow >= 0) lu[c.hrow * LZ + R + c.lane] = yh[j];
+        lds_barrier();
+        const int x = x0 + j, xp = x * pl;
+        T f = fv[j];
+        if (MODE == 3) f += mm[j] * (-dv[j]);
+        T lx = T(0), lyy = T(0), lz = T(0);
+#pragma unroll
+        for (int i = R; i >= 1; --i) {
+            lx = fma(P.cx.d2[i], q[R + j + i] + q[R + j - i], lx);
+            lyy = fma(P.cy.d2[i], lu[own + i * LZ] + lu[own - i * LZ], lyy);
+            lz = fma(P.cz.d2[i], lu[own + i] + lu[own - i], lz);
+        }
+        lx = fma(P.cx.d2[0], v, lx);
+        lyy = fma(P.cy.d2[0], v, lyy);
+        lz = fma(P.cz.d2[0], v, lz);
+        const T lap = (lx + lyy) + lz;
+        const T c2 = cv[j] * cv[j];
+        const T src = f + lap;
+        const T up = fma(P.tc.dt2, c2 * src, T(2) * v - um[j]);
+        if (MODE == 2) buf_st(c.ic, at(ovi, xp * (8 / SZ)), c.shb8, icv[j] + (double)v * (double)dv[j]);
+        buf_st(c.uo, at(ov, xp), 0, up);
+        if (MODE == 1) buf_st(c.dw, at(ovd, xp), c.shb, c2 * src);
+        if (c.has_fst) buf_st(c.fst, at(ovf, xp), c.shb, v);
+    }
+#undef PA_Q
+#undef PA_ROW
+}
+
+// ---- PML tiles -----------------------------------------------------------------
+template <typename T, int R, int MODE>
+__device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DParams<T> &P, Lds3D<T, R> &L,
+                                            int x0, int hl_pz)
+{
+    constexpr int TX = TX3, D = PA3D_PREFETCH, NQ = TX + 2 * R, LZ = Lds3D<T, R>::LZ;
+    constexpr int SZ = sizeof(T);
+    const int pl = c.plane, nx = c.nx;
+    const int co = c.co, cho = c.cho, hro = c.hro, ov = c.ov, mv = c.mv;
+    const int ovd = c.ovd, ovi = c.ovi, ovf = c.ovf;
+    T q[NQ], qx[NQ], qs[NQ];
+    T uz[TX], yh[TX], sz[TX], sh[TX], pyc[TX], pyh[TX], pzc[TX], pzz[TX];
+    T um[TX], cv[TX], fv[TX], dv[TX], mm[TX];
+    double icv[TX];
+#define PA_Q(s_)                                                                                    \
+    do {                                                                                            \
+        const int xx = x0 - R + (s_);                                                               \
+        const bool in = xx >= 0 && xx < nx;                                                         \
+        const int o = in ? at(co, xx * pl) : kOOB;                                                  \
+        q[s_] = buf_ld<T>(c.u, o, 0);                                                               \
+        qx[s_] = buf_ld<T>(c.px, o, 0);                                                             \
+        qs[s_] = buf_ld<T>(c.ps, o, 0);                                                             \
+    } while (0)
+#define PA_ROW(r_)                                                                                  \
+    do {                                                                                            \
+        const int r = (r_), x = x0 + r;                                                             \
+        const bool in = x < nx;                                                                     \
+        const int xs = in ? x : nx - 1, xp = xs * pl;                                              \
+        const int oc = in ? at(co, xp) : kOOB, oz = in ? at(cho, xp) : kOOB;                        \
+        const int oh = in ? at(hro, xp) : kOOB, oo = in ? at(ov, xp) : kOOB;                        \
+        uz[r] = buf_ld<T>(c.u, oz, 0);                                                              \
+        yh[r] = buf_ld<T>(c.u, oh, 0);                                                              \
+        sz[r] = buf_ld<T>(c.ps, oz, 0);                                                             \
+        sh[r] = buf_ld<T>(c.ps, oh, 0);                                                             \
+        pyc[r] = buf_ld<T>(c.py, oc, 0);                                                            \
+        pyh[r] = buf_ld<T>(c.py, oh, 0);                                                            \
+        pzc[r] = buf_ld<T>(c.pz, oc, 0);                                                            \
+        pzz[r] = buf_ld<T>(c.pz, oz, 0);                                                            \
+        um[r] = buf_ld<T>(c.uo, oo, 0);                                                             \
+        const int mp = clampi(xs - c.mx_off, c.mx_lo, c.mx_hi) * c.mplane;                         \
+        cv[r] = buf_ld<T>(c.model, mv, mp);                                                         \
+        T f_ = c.has_rhs ? buf_ld<T>(c.rhs, oo, 0) : T(0);                                          \
+        f_ += c.finj[r * c.fstride];                                                                \
+        fv[r] = f_;                                                                                 \
+        dv[r] = (MODE >= 2) ? buf_ld<T>(c.dw, in ? at(ovd, xp) : kOOB, c.shb) : T(0);               \
+        mm[r] = (MODE == 3) ? buf_ld<T>(c.m1, mv, mp) : T(0);                                       \
+        icv[r] = (MODE == 2) ? buf_ld<double>(c.ic, in ? at(ovi, xp * (8 / SZ)) : kOOB, c.shb8) : 0.0; \
+    } while (0)
+#pragma unroll
+    for (int s = 0; s < 2 * R + D && s < NQ; ++s) PA_Q(s);
+#pragma unroll
+    for (int s = 0; s < D && s < TX; ++s) PA_ROW(s);
+    __builtin_amdgcn_sched_barrier(0);
+    const T syk = c.syk, sym = c.sym, szk = c.szk, szm = c.szm, vsig = c.vsig;
+    const int own = (R + c.w) * LZ + R + c.lane;     // u, psi planes
+    const int owny = (R + c.w) * 64 + c.lane;        // Phiy plane
+    const int ownz = c.w * LZ + R + c.lane;          // Phiz plane
+#pragma unroll
+    for (int j = 0; j < TX; ++j) {
+        if (j + 2 * R + D < NQ) PA_Q(j + 2 * R + D);
+        if (j + D < TX) PA_ROW(j + D);
+        __builtin_amdgcn_sched_barrier(0);
+        const int b = j & 1;
+        T *lu = L.u[b], *lps = L.ps[b], *lpy = L.py[b], *lpz = L.pz[b];
+        const T uc = q[R + j], psc = qs[R + j];
+        lu[own] = uc;
+        lps[own] = psc;
+        lpy[owny] = pyc[j];
+        lpz[ownz] = pzc[j];
+        st_halo<T, R>(lu, L.junk[0], c.hl, c.lane, uz[j]);
+        st_halo<T, R>(lps, L.junk[1], c.hl, c.lane, sz[j]);
+        st_halo<T, R>(lpz, L.junk[2], hl_pz, c.lane, pzz[j]);
+        if (c.hrow >= 0) {
+            lu[c.hrow * LZ + R + c.lane] = yh[j];
+            lps[c.hrow * LZ + R + c.lane] = sh[j];
+            lpy[c.hrow * 64 + c.lane] = pyh[j];
+        }
+        lds_barrier();
+        const int x = x0 + j;
+        const bool inx = x < nx;
+        const int xp = (inx ? x : nx - 1) * pl;
+        T f = fv[j];
+        if (MODE == 3) f += mm[j] * (-dv[j]);
+        T lx = T(0), lyy = T(0), lz = T(0);
+#pragma unroll
+        for (int i = R; i >= 1; --i) {
+            lx = fma(P.cx.d2[i], q[R + j + i] + q[R + j - i], lx);
+            lyy = fma(P.cy.d2[i], lu[own + i * LZ] + lu[own - i * LZ], lyy);
+            lz = fma(P.cz.d2[i], lu[own + i] + lu[own - i], lz);
+        }
+        lx = fma(P.cx.d2[0], uc, lx);
+        lyy = fma(P.cy.d2[0], uc, lyy);
+        lz = fma(P.cz.d2[0], uc, lz);
+        const T lap = (lx + lyy) + lz;
+        const T c2 = cv[j] * cv[j];
+        const T u_m = um[j];
+        const bool edge = (x == 0) | (x == nx - 1) | c.yzedge;
+        T dux = T(0), duy = T(0), duz = T(0), dpx = T(0), dpy = T(0), dpz = T(0);
+        T dsx = T(0), dsy = T(0), dsz = T(0);
+#pragma unroll
+        for (int i = R; i >= 1; --i) {
+            const T ax = P.cx.d1[i - 1], ay = P.cy.d1[i - 1], az = P.cz.d1[i - 1];
+            dux = fma(ax, q[R + j + i] - q[R + j - i], dux);
+            dpx = fma(ax, qx[R + j + i] - qx[R + j - i], dpx);
+            dsx = fma(ax, qs[R + j + i] - qs[R + j - i], dsx);
+            duy = fma(ay, lu[own + i * LZ] - lu[own - i * LZ], duy);
+            dpy = fma(ay, lpy[owny + i * 64] - lpy[owny - i * 64], dpy);
+            dsy = fma(ay, lps[own + i * LZ] - lps[own - i * LZ], dsy);
+            duz = fma(az, lu[own + i] - lu[own - i], duz);
+            dpz = fma(az, lpz[ownz + i] - lpz[ownz - i], dpz);
+            dsz = fma(az, lps[own + i] - lps[own - i], dsz);
+        }
+        const T src = (((f + lap) + dpx) + dpy) + dpz;
+        const T sxk = bcast3(vsig, j), sxm = bcast3(vsig, TX + j);
+        const T dt = P.tc.dt;
+        T psn = psc + dt * uc;
+        const T s1k = (sxk + syk) + szk;
+        const T s2k = (sxk * syk + syk * szk) + szk * sxk;
+        const T s3k = (sxk * syk) * szk;
+        const bool pml = sxk != T(0) || syk != T(0) || szk != T(0);
+        const T pxc = qx[R + j], pycj = pyc[j], pzcj = pzc[j];
+        const T pxd = (pxc - dt * sxk * pxc) + dt * ((syk + szk) - sxk) * dux + dt * (syk * szk * dsx);
+        const T pyd = (pycj - dt * syk * pycj) + dt * ((szk + sxk) - syk) * duy + dt * (szk * sxk * dsy);
+        const T pzd = (pzcj - dt * szk * pzcj) + dt * ((sxk + syk) - szk) * duz + dt * (sxk * syk * dsz);
+        const T fac1 = P.tc.two_dt2 / (T(2) + dt * s1k);
+        const T fac2 = c2 * (src - s3k * psc) - (u_m - T(2) * uc) * P.tc.inv_dt2 + s1k * u_m * P.tc.inv_2dt -
+                       s2k * uc;
+        const T upd = fac1 * fac2;
+        const T upi = fma(P.tc.dt2, c2 * src, T(2) * uc - u_m);
+        T pxn = pml ? pxd : pxc;
+        T pyn = pml ? pyd : pycj;
+        T pzn = pml ? pzd : pzcj;
+        T up = pml ? upd : upi;
+        const T s1m = (sxm + sym) + szm;
+        const T s2m = (sxm * sym + sxm * szm) + sym * szm;
+        const T s3m = (sxm * sym) * szm;
+        T dw = c2 * (src - s3k * psc) + (s1m - s1k) * (up - u_m) * P.tc.inv_2dt + (s2m - s2k) * uc + s3m * psc;
+        if (edge) {
+            up = T(0);
+            dw = T(0);
+            pxn = T(0);
+            pyn = T(0);
+            pzn = T(0);
+            psn = T(0);
+        }
+        const int oo = inx ? at(ov, xp) : kOOB;
+        if (MODE == 2) buf_st(c.ic, inx ? at(ovi, xp * (8 / SZ)) : kOOB, c.shb8, icv[j] + (double)uc * (double)dv[j]);
+        buf_st(c.uo, oo, 0, up);
+        buf_st(c.pxo, oo, 0, pxn);
+        buf_st(c.pyo, oo, 0, pyn);
+        buf_st(c.pzo, oo, 0, pzn);
+        buf_st(c.pso, oo, 0, psn);
+        if (MODE == 1) buf_st(c.dw, inx ? at(ovd, xp) : kOOB, c.shb, dw);
+        if (c.has_fst) buf_st(c.fst, inx ? at(ovf, xp) : kOOB, c.shb, uc);
+    }
+#undef PA_Q
+#undef PA_ROW
+}
 
 template <typename T, int R, int MODE>
 __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DParams<T> &P,
-                                       const pysit_amd_shot &S, int tile, int flags, int tloc,
-                                       int slot, int seg_base, Planes3D<T, R> &L, T *lds_f)
+                                       const pysit_amd_shot &S, int tile, int full, int tloc, int slot,
+                                       int seg_base, Lds3D<T, R> &L)
 {
+    constexpr int SZ = sizeof(T), TX = TX3, LZ = Lds3D<T, R>::LZ;
     const int k = slot - S.kshift;
     if (k < 0 || k >= S.nt) return;
-    constexpr int LZ = 64 + 2 * R;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, ty = tid >> 6;
+    Ctx3D<T, R> c;
     const int nx = S.nx, ny = S.ny, nz = S.nz;
-    const int tz = tloc % S.ntz;
-    const int tyi = (tloc / S.ntz) % S.nty;
-    const int tx = tloc / (S.ntz * S.nty);
-    const int z0 = tz * 64, y0 = tyi * TY3;
-    const int z = z0 + lane, y = y0 + ty;
-    const int x0 = tx * TX3, x1 = min(x0 + TX3, nx);
-    const bool valid = (z < nz) && (y < ny);
-    const bool full = flags & 1;
+    c.nx = nx;
+    c.ny = ny;
+    c.nz = nz;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    c.lane = lane;
+    c.w = w;
+    // tiles: z fastest, then y, then x; z tiles as in 2D (the last ends on nz-1)
+    const int ntz = S.ntz;
+    const int tz = tloc % ntz, tyi = (tloc / ntz) % S.nty, tx = tloc / (ntz * S.nty);
+    const int zlast = nz > 64 ? nz - 64 : 0;
+    const int z0 = (tz == ntz - 1) ? zlast : tz * 64;
+    const int own_hi = (tz == ntz - 1) ? nz : min(z0 + 64, zlast);
+    const int y0 = tyi * TY3, x0 = tx * TX;
+    const int z = z0 + lane, y = y0 + w;
+    c.y = y;
+    const bool zin = z < nz, yin = y < ny;
+    const bool own = zin && z < own_hi && yin;
+    c.plane = ny * nz * SZ;
+    c.co = (zin && yin) ? (y * nz + z) * SZ : kOOB;
+    c.ov = own ? (y * nz + z) * SZ : kOOB;
+    c.ov8 = own ? (y * nz + z) * 8 : kOOB;
+    const int hz = lane < R ? z0 + 64 + lane : (lane >= 64 - R ? z0 + lane - 64 : -1);
+    const bool hin = hz >= 0 && hz < nz;
+    c.cho = (hin && yin) ? (y * nz + hz) * SZ : kOOB;
+    // LDS column of the z-halo value, own row in the u/psi planes and the Phiz plane
+    const int hcol = lane < R ? R + 64 + lane : (lane >= 64 - R ? lane - (64 - R) : -1);
+    c.hl = hcol >= 0 ? (R + w) * LZ + hcol : -1;
+    const int hl_pz = hcol >= 0 ? w * LZ + hcol : -1;
+    // y-halo rows: waves 0..r-1 above the tile, waves r..2r-1 below it
+    c.hrow = w < R ? w : (w < 2 * R ? TY3 + w : -1);
+    const int hy = y0 - R + c.hrow;
+    c.hro = (c.hrow >= 0 && hy >= 0 && hy < ny && zin) ? (hy * nz + z) * SZ : kOOB;
+    c.yzedge = y == 0 || y >= ny - 1 || z == 0 || z >= nz - 1;
     const int taken = (MODE == 2) ? (S.nt - 1 - k) : k;
     const int cur = taken & 1;
     const long long off = S.off;
-    const long long sx_ = (long long)ny * nz;  // x stride
-    const T *__restrict__ u = (const T *)B.u[cur] + off;
-    T *__restrict__ uo = (T *)B.u[cur ^ 1] + off;
-    const T *__restrict__ px = (const T *)B.phix[cur] + off;
-    T *__restrict__ pxo = (T *)B.phix[cur ^ 1] + off;
-    const T *__restrict__ py = (const T *)B.phiy[cur] + off;
-    T *__restrict__ pyo = (T *)B.phiy[cur ^ 1] + off;
-    const T *__restrict__ pz = (const T *)B.phiz[cur] + off;
-    T *__restrict__ pzo = (T *)B.phiz[cur ^ 1] + off;
-    const T *__restrict__ ps = (const T *)B.psi[cur] + off;
-    T *__restrict__ pso = (T *)B.psi[cur ^ 1] + off;
-    const T *__restrict__ model = (const T *)S.model;
-    const long long myz = (long long)clampi(y - S.my_off, S.my_lo, S.my_hi) * S.mnz +
-                          clampi(z - S.mz_off, S.mz_lo, S.mz_hi);
-    const T *xl = (const T *)S.xl, *xr = (const T *)S.xr;
-    const T *yl = (const T *)S.yl, *yr = (const T *)S.yr;
-    const T *zl = (const T *)S.zl, *zr = (const T *)S.zr;
-
-    T syk = T(0), sym = T(0), szk = T(0), szm = T(0);
-    if (full && valid) {
-        syk = (y <= ny - 2) ? sigma_kernel<T>(yl, S.nyl, yr, S.nyr, y, ny) : T(0);
-        sym = sigma_mesh<T>(yl, S.nyl, yr, S.nyr, y, ny);
-        szk = (z <= nz - 2) ? sigma_kernel<T>(zl, S.nzl, zr, S.nzr, z, nz) : T(0);
-        szm = sigma_mesh<T>(zl, S.nzl, zr, S.nzr, z, nz);
-    }
+    const long long nb = (long long)nx * ny * nz * SZ;
+    const T *vb = (const T *)B.u[0];
+    c.u = make_rsrc((const T *)B.u[cur] + off, nb);
+    c.uo = make_rsrc((T *)B.u[cur ^ 1] + off, nb);
+    c.px = make_rsrc((const T *)B.phix[cur] + off, full ? nb : 0);
+    c.pxo = make_rsrc((T *)B.phix[cur ^ 1] + off, full ? nb : 0);
+    c.py = make_rsrc((const T *)B.phiy[cur] + off, full ? nb : 0);
+    c.pyo = make_rsrc((T *)B.phiy[cur ^ 1] + off, full ? nb : 0);
+    c.pz = make_rsrc((const T *)B.phiz[cur] + off, full ? nb : 0);
+    c.pzo = make_rsrc((T *)B.phiz[cur ^ 1] + off, full ? nb : 0);
+    c.ps = make_rsrc((const T *)B.psi[cur] + off, full ? nb : 0);
+    c.pso = make_rsrc((T *)B.psi[cur ^ 1] + off, full ? nb : 0);
+    c.model = make_rsrc(S.model, (long long)(S.mx_hi + 1) * S.mny * S.mnz * SZ);
+    c.mv = (clampi(y - S.my_off, S.my_lo, S.my_hi) * S.mnz + clampi(z - S.mz_off, S.mz_lo, S.mz_hi)) * SZ;
+    c.mplane = S.mny * S.mnz * SZ;
+    c.mx_off = S.mx_off;
+    c.mx_lo = S.mx_lo;
+    c.mx_hi = S.mx_hi;
+    c.has_rhs = P.rhs != nullptr;
+    c.rhs = make_rsrc(c.has_rhs ? (const void *)P.rhs : vb, c.has_rhs ? nb : 0);
+    c.shb = (int)(off * SZ);
+    c.shb8 = (int)(off * 8);
     const bool img = (MODE == 2) && (k % B.imaging_period == 0) && B.dw;
     const bool store = (MODE == 1) && (k % B.imaging_period == 0) && B.dw;
-    const long long soff = (long long)(slot - seg_base) * B.total_n + off;
-    const T *dwi = (img || MODE == 3) ? (const T *)B.dw + soff : nullptr;
-    T *dwo = store ? (T *)B.dw + soff : nullptr;
-    T *fst = B.fstore ? (T *)B.fstore + soff : nullptr;
-    double *ic = (MODE == 2) ? B.ic + off : nullptr;
-    const T *m1 = (MODE == 3) ? (const T *)B.born_m1 : nullptr;
-
-    const long long yz = (long long)y * nz + z;
-    T q[2 * R + 1], qx[2 * R + 1], qs[2 * R + 1];
-#pragma unroll
-    for (int j = 0; j < 2 * R; ++j) {
-        const int xx = x0 - R + j;
-        const bool in = valid && xx >= 0 && xx < nx;
-        q[j] = in ? u[xx * sx_ + yz] : T(0);
-        qx[j] = (full && in) ? px[xx * sx_ + yz] : T(0);
-        qs[j] = (full && in) ? ps[xx * sx_ + yz] : T(0);
+    const long long slot_el = (long long)(slot - seg_base) * B.total_n;
+    const long long slot_bytes = B.total_n * SZ;
+    const bool dwon = ((img || MODE == 3) && B.dw) || store;
+    c.dw = make_rsrc(dwon ? (const void *)((const T *)B.dw + slot_el) : vb, dwon ? slot_bytes : 0);
+    c.has_fst = B.fstore != nullptr;
+    c.fst = make_rsrc(c.has_fst ? (const void *)((const T *)B.fstore + slot_el) : vb, c.has_fst ? slot_bytes : 0);
+    c.ic = make_rsrc(MODE == 2 ? (const void *)B.ic : vb, MODE == 2 ? B.total_n * 8 : 0);
+    c.m1 = make_rsrc(MODE == 3 ? B.born_m1 : vb,
+                     MODE == 3 ? (long long)(S.mx_hi + 1) * S.mny * S.mnz * SZ : 0);
+    c.ovd = dwon ? c.ov : kOOB;
+    c.ovi = img ? c.ov8 : kOOB;
+    c.ovf = c.has_fst ? c.ov : kOOB;
+    // PML profiles: lane y/z values and the tile planes' x profile (lane p:
+    // kernel form of plane x0+p, lane TX+p: mesh form), masked buffer loads
+    static_assert(2 * TX <= 64, "the x profile of a tile must fit one wavefront");
+    {
+        const Rsrc rxl = make_rsrc(S.nxl ? S.xl : vb, (long long)S.nxl * SZ);
+        const Rsrc rxr = make_rsrc(S.nxr ? S.xr : vb, (long long)S.nxr * SZ);
+        const Rsrc ryl = make_rsrc(S.nyl ? S.yl : vb, (long long)S.nyl * SZ);
+        const Rsrc ryr = make_rsrc(S.nyr ? S.yr : vb, (long long)S.nyr * SZ);
+        const Rsrc rzl = make_rsrc(S.nzl ? S.zl : vb, (long long)S.nzl * SZ);
+        const Rsrc rzr = make_rsrc(S.nzr ? S.zr : vb, (long long)S.nzr * SZ);
+        const bool f = full && zin && yin;
+        c.syk = sigma3<T>(ryl, S.nyl, ryr, S.nyr, f && y <= ny - 2 ? y : -1, ny, true);
+        c.sym = sigma3<T>(ryl, S.nyl, ryr, S.nyr, f ? y : -1, ny, false);
+        c.szk = sigma3<T>(rzl, S.nzl, rzr, S.nzr, f && z <= nz - 2 ? z : -1, nz, true);
+        c.szm = sigma3<T>(rzl, S.nzl, rzr, S.nzr, f ? z : -1, nz, false);
+        const int xs = min(x0 + (lane < TX ? lane : lane - TX), nx - 1);
+        const int xi = (full && lane < 2 * TX) ? (lane < TX ? max(0, min(xs, nx - 2)) : xs) : -1;
+        c.vsig = sigma3<T>(rxl, S.nxl, rxr, S.nxr, xi, nx, lane < TX);
     }
-    q[2 * R] = qx[2 * R] = qs[2 * R] = T(0);
-
-    // sources of this tile, gathered into LDS (x-plane, y row, z lane): one
-    // thread per destination node, entries summed in the reference's order
+    // sources of this tile gathered into LDS (plane, row, lane): one thread
+    // per destination node, entries summed in the reference's order; without
+    // sources every plane reads a zero slot
+    c.finj = &L.junk[3][0];
+    c.fstride = 0;
     bool has_inj = false;
     if (B.inj_ptr) {
         const int n0 = B.inj_ptr[tile], n1 = B.inj_ptr[tile + 1];
-        if (n1 > n0) {
+        if (n1 > n0) has_inj = true;
+        if (has_inj) {
             const double *amp_k = S.amp + (long long)k * S.namp;
-            for (int i = tid; i < TX3 * TY3 * 64; i += TY3 * 64) lds_f[i] = T(0);
-            __syncthreads();
-            for (int n = n0 + tid; n < n1; n += TY3 * 64) {
+            for (int i = threadIdx.x; i < TX3 * TY3 * 64; i += TY3 * 64) L.f[i] = T(0);
+            lds_barrier();
+            for (int n = n0 + (int)threadIdx.x; n < n1; n += TY3 * 64) {
                 double acc = 0.0;
-                for (int e = B.inj_nptr[n]; e < B.inj_nptr[n + 1]; ++e)
-                    acc += B.inj_w[e] * amp_k[B.inj_chan[e]];
-                lds_f[B.inj_node[n]] = (T)acc;
+                for (int e = B.inj_nptr[n]; e < B.inj_nptr[n + 1]; ++e) acc += B.inj_w[e] * amp_k[B.inj_chan[e]];
+                L.f[B.inj_node[n]] = (T)acc;
             }
-            __syncthreads();
-            has_inj = true;
+            c.finj = &L.f[w * 64 + lane];
+            c.fstride = TY3 * 64;
         }
     }
-
-    // halo elements of an LY x LZ plane outside the TY3 x 64 centre
-    constexpr int HU = 2 * R * LZ + TY3 * 2 * R;
-    auto halo_uz = [&](int h, int &ly, int &lz) {
-        if (h < 2 * R * LZ) {
-            const int rr = h / LZ;
-            ly = rr < R ? rr : rr + TY3;
-            lz = h % LZ;
-        } else {
-            const int hh = h - 2 * R * LZ;
-            ly = R + hh / (2 * R);
-            const int c = hh % (2 * R);
-            lz = c < R ? c : c + 64;
-        }
-    };
-
-    for (int x = x0; x < x1; ++x) {
-        const int xa = x + R;
-        const bool ain = valid && xa < nx;
-        q[2 * R] = ain ? u[xa * sx_ + yz] : T(0);
-        if (full) {
-            qx[2 * R] = ain ? px[xa * sx_ + yz] : T(0);
-            qs[2 * R] = ain ? ps[xa * sx_ + yz] : T(0);
-        }
-        const long long pl = (long long)x * sx_;
-        const T uc = q[R];
-        L.u[R + ty][R + lane] = uc;
-        T pyc = T(0), pzc = T(0);
-        if (full) {
-            L.ps[R + ty][R + lane] = qs[R];
-            pyc = valid ? py[pl + yz] : T(0);
-            pzc = valid ? pz[pl + yz] : T(0);
-            L.py[R + ty][lane] = pyc;
-            L.pz[ty][R + lane] = pzc;
-        }
-        for (int h = tid; h < HU; h += TY3 * 64) {
-            int ly, lz;
-            halo_uz(h, ly, lz);
-            const int gy = y0 - R + ly, gz = z0 - R + lz;
-            const bool in = gy >= 0 && gy < ny && gz >= 0 && gz < nz;
-            const long long g = pl + (long long)gy * nz + gz;
-            L.u[ly][lz] = in ? u[g] : T(0);
-            if (full) L.ps[ly][lz] = in ? ps[g] : T(0);
-        }
-        if (full) {
-            for (int h = tid; h < 2 * R * 64; h += TY3 * 64) {
-                const int rr = h / 64, c = h % 64;
-                const int ly = rr < R ? rr : rr + TY3;
-                const int gy = y0 - R + ly, gz = z0 + c;
-                const bool in = gy >= 0 && gy < ny && gz < nz;
-                L.py[ly][c] = in ? py[pl + (long long)gy * nz + gz] : T(0);
-            }
-            for (int h = tid; h < TY3 * 2 * R; h += TY3 * 64) {
-                const int rr = h / (2 * R), c = h % (2 * R);
-                const int lz = c < R ? c : c + 64;
-                const int gy = y0 + rr, gz = z0 - R + lz;
-                const bool in = gy < ny && gz >= 0 && gz < nz;
-                L.pz[rr][lz] = in ? pz[pl + (long long)gy * nz + gz] : T(0);
-            }
-        }
-        T f = has_inj ? lds_f[((x - x0) * TY3 + ty) * 64 + lane] : T(0);
-        if (P.rhs && valid) f += P.rhs[pl + yz];
-        const long long mi = (long long)clampi(x - S.mx_off, S.mx_lo, S.mx_hi) * S.mny * S.mnz + myz;
-        if (MODE == 3 && valid) f += m1[mi] * (-dwi[pl + yz]);
-        const T um = valid ? uo[pl + yz] : T(0);
-        const T cv = model[mi];
-        __syncthreads();
-
-        if (valid) {
-            T lx = T(0), lyy = T(0), lz = T(0);
-#pragma unroll
-            for (int j = R; j >= 1; --j) {
-                lx = fma(P.cx.d2[j], q[R + j] + q[R - j], lx);
-                lyy = fma(P.cy.d2[j], L.u[R + ty + j][R + lane] + L.u[R + ty - j][R + lane], lyy);
-                lz = fma(P.cz.d2[j], L.u[R + ty][R + lane + j] + L.u[R + ty][R + lane - j], lz);
-            }
-            lx = fma(P.cx.d2[0], uc, lx);
-            lyy = fma(P.cy.d2[0], uc, lyy);
-            lz = fma(P.cz.d2[0], uc, lz);
-            const T lap = (lx + lyy) + lz;
-            const T c2 = cv * cv;
-            const bool edge = (x == 0) | (x == nx - 1) | (y == 0) | (y == ny - 1) | (z == 0) |
-                              (z == nz - 1);
-            T up, dw, pxn = T(0), pyn = T(0), pzn = T(0), psn = T(0);
-            if (!full) {
-                const T src = f + lap;
-                up = fma(P.tc.dt2, c2 * src, T(2) * uc - um);
-                dw = c2 * src;
-            } else {
-                T dux = T(0), duy = T(0), duz = T(0), dpx = T(0), dpy = T(0), dpz = T(0);
-                T dsx = T(0), dsy = T(0), dsz = T(0);
-#pragma unroll
-                for (int j = R; j >= 1; --j) {
-                    const T ax = P.cx.d1[j - 1], ay = P.cy.d1[j - 1], az = P.cz.d1[j - 1];
-                    dux = fma(ax, q[R + j] - q[R - j], dux);
-                    dpx = fma(ax, qx[R + j] - qx[R - j], dpx);
-                    dsx = fma(ax, qs[R + j] - qs[R - j], dsx);
-                    duy = fma(ay, L.u[R + ty + j][R + lane] - L.u[R + ty - j][R + lane], duy);
-                    dpy = fma(ay, L.py[R + ty + j][lane] - L.py[R + ty - j][lane], dpy);
-                    dsy = fma(ay, L.ps[R + ty + j][R + lane] - L.ps[R + ty - j][R + lane], dsy);
-                    duz = fma(az, L.u[R + ty][R + lane + j] - L.u[R + ty][R + lane - j], duz);
-                    dpz = fma(az, L.pz[ty][R + lane + j] - L.pz[ty][R + lane - j], dpz);
-                    dsz = fma(az, L.ps[R + ty][R + lane + j] - L.ps[R + ty][R + lane - j], dsz);
-                }
-                const T src = (((f + lap) + dpx) + dpy) + dpz;
-                const T sxk = sigma_kernel<T>(xl, S.nxl, xr, S.nxr, min(x, nx - 2), nx);
-                const T sxm = sigma_mesh<T>(xl, S.nxl, xr, S.nxr, x, nx);
-                const T dt = P.tc.dt;
-                const T psc = qs[R];
-                psn = psc + dt * uc;
-                const T s1k = (sxk + syk) + szk;
-                const T s2k = (sxk * syk + syk * szk) + szk * sxk;
-                const T s3k = (sxk * syk) * szk;
-                if (sxk != T(0) || syk != T(0) || szk != T(0)) {
-                    const T pxc = qx[R];
-                    pxn = (pxc - dt * sxk * pxc) + dt * ((syk + szk) - sxk) * dux + dt * (syk * szk * dsx);
-                    pyn = (pyc - dt * syk * pyc) + dt * ((szk + sxk) - syk) * duy + dt * (szk * sxk * dsy);
-                    pzn = (pzc - dt * szk * pzc) + dt * ((sxk + syk) - szk) * duz + dt * (sxk * syk * dsz);
-                    const T fac1 = P.tc.two_dt2 / (T(2) + dt * s1k);
-                    const T fac2 = c2 * (src - s3k * psc) - (um - T(2) * uc) * P.tc.inv_dt2 +
-                                   s1k * um * P.tc.inv_2dt - s2k * uc;
-                    up = fac1 * fac2;
-                } else {
-                    pxn = qx[R];
-                    pyn = pyc;
-                    pzn = pzc;
-                    up = fma(P.tc.dt2, c2 * src, T(2) * uc - um);
-                }
-                const T s1m = (sxm + sym) + szm;
-                const T s2m = (sxm * sym + sxm * szm) + sym * szm;
-                const T s3m = (sxm * sym) * szm;
-                dw = c2 * (src - s3k * psc) + (s1m - s1k) * (up - um) * P.tc.inv_2dt +
-                     (s2m - s2k) * uc + s3m * psc;
-            }
-            if (edge) {
-                up = dw = pxn = pyn = pzn = psn = T(0);
-            }
-            const long long g = pl + yz;
-            if (MODE == 2 && img) ic[g] += (double)uc * (double)dwi[g];
-            uo[g] = up;
-            if (full) {
-                pxo[g] = pxn;
-                pyo[g] = pyn;
-                pzo[g] = pzn;
-                pso[g] = psn;
-            }
-            if (MODE == 1 && store) dwo[g] = dw;
-            if (fst) fst[g] = uc;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < 2 * R; ++j) {
-            q[j] = q[j + 1];
-            qx[j] = qx[j + 1];
-            qs[j] = qs[j + 1];
-        }
-    }
+    if (!has_inj && threadIdx.x == 0) L.junk[3][0] = T(0);
+    lds_barrier();
+    if (full) march3_full<T, R, MODE>(c, P, L, x0, hl_pz);
+    else march3_int<T, R, MODE>(c, P, L, x0);
 }
 
 // Receiver sampling, 16 lanes per receiver with a fixed reduction order
@@ -314,23 +510,22 @@ __device__ __forceinline__ void sample_group3(const pysit_amd_batch &B, int slot
 template <typename T, int R, int MODE>
 __global__ __launch_bounds__(512) void k_step3d(const pysit_amd_batch B, const Step3DParams<T> P,
                                                 int slot, int seg_base, int n_tile_blocks,
-                                                int n_rec_lanes)
+                                                int n_rec_lanes, int rec_blocks)
 {
-    __shared__ Planes3D<T, R> L;
-    __shared__ T lds_f[TX3 * TY3 * 64];
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
-    if (blk < n_tile_blocks) {
-        if (P.single) {
-            tile3d<T, R, MODE>(B, P, P.shot0, blk, 1, blk, slot, seg_base, L, lds_f);
-        } else {
-            const int4 td = reinterpret_cast<const int4 *>(B.tiles)[blk];
-            tile3d<T, R, MODE>(B, P, B.shots[td.x], blk, td.z, td.y, slot, seg_base, L, lds_f);
-        }
+    __shared__ Lds3D<T, R> L;
+    const int b = blockIdx.x;
+    if (MODE != 2 && b < rec_blocks) {
+        const int rec = b * (TY3 * 4) + (threadIdx.x >> 4);
+        sample_group3<T>(B, slot, rec < n_rec_lanes ? rec : B.n_rec, threadIdx.x & 15);
         return;
     }
-    if (MODE != 2) {
-        const int rec = (blk - n_tile_blocks) * (TY3 * 4) + (threadIdx.x >> 4);
-        sample_group3<T>(B, slot, rec < n_rec_lanes ? rec : B.n_rec, threadIdx.x & 15);
+    const int blk = xcd_block(b - rec_blocks, gridDim.x - rec_blocks);
+    if (blk >= n_tile_blocks) return;
+    if (P.single) {
+        tile3d<T, R, MODE>(B, P, P.shot0, blk, 1, blk, slot, seg_base, L);
+    } else {
+        const int4 td = reinterpret_cast<const int4 *>(B.tiles)[blk];
+        tile3d<T, R, MODE>(B, P, B.shots[td.x], blk, td.z & 1, td.y, slot, seg_base, L);
     }
 }
 
@@ -338,11 +533,12 @@ template <typename T, int R, int MODE>
 static int launch3d(const pysit_amd_batch &B, const Step3DParams<T> &P, int slot, int seg_base,
                     int n_tiles, int n_recs, hipStream_t st)
 {
-    const int rec_blocks = (MODE != 2) ? (n_recs + TY3 * 4 - 1) / (TY3 * 4) : 0;
+    const int recs = (MODE != 2) ? n_recs : 0;
+    const int rec_blocks = recs ? (recs + TY3 * 4 - 1) / (TY3 * 4) : 0;
     const int blocks = n_tiles + rec_blocks;
     if (blocks == 0) return PYSIT_AMD_OK;
     hipLaunchKernelGGL((k_step3d<T, R, MODE>), dim3(blocks), dim3(TY3 * 64), 0, st, B, P, slot,
-                       seg_base, n_tiles, (MODE != 2) ? n_recs : 0);
+                       seg_base, n_tiles, recs, rec_blocks);
     PA_HIP(hipGetLastError());
     return PYSIT_AMD_OK;
 }

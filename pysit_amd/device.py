@@ -31,9 +31,9 @@ TILE_Z = 64      # z nodes per wavefront (the contiguous axis)
 
 
 def z_tiles_2d(nz):
-    """2D z tiles: (start, owned end) per tile.  Tiles start at 0, 64, 128, ...
-    except the last, which ends on node nz-1; each node is owned by exactly one
-    tile (the kernel's tiling, cda2d.hip tile2d)."""
+    """z tiles (2D and 3D): (start, owned end) per tile.  Tiles start at 0, 64,
+    128, ... except the last, which ends on node nz-1; each node is owned by
+    exactly one tile (the kernels' tiling, cda2d.hip tile2d, cda3d.hip tile3d)."""
     ntz = -(-nz // TILE_Z)
     zlast = max(0, nz - TILE_Z)
     out = []
@@ -193,20 +193,20 @@ class DeviceBatch:
                     flags[tx * ntz + tz] = 1 if (xp or zp) else 0
             return (ntx, 1, ntz), flags
         nx, ny, nz = s.shape
-        ntz = -(-nz // TILE_Z)
+        zt = z_tiles_2d(nz)
+        ntz = len(zt)
         nty = -(-ny // self.tile_y)
         ntx = -(-nx // self.tile_x)
         (xl, xr), (yl, yr), (zl, zr) = s.kpml
         flags = np.zeros(ntx * nty * ntz, dtype=np.int32)
         for tx in range(ntx):
             x0, x1 = tx * self.tile_x, min(nx, (tx + 1) * self.tile_x) - 1
-            xp = (x0 - R < xl.size) or (x1 + R >= nx - xr.size)
+            xp = (x0 - R < max(xl.size, 1)) or (x1 + R > nx - 1 - max(xr.size, 1))
             for ty in range(nty):
                 y0, y1 = ty * self.tile_y, min(ny, (ty + 1) * self.tile_y) - 1
-                yp = (y0 - R < yl.size) or (y1 + R >= ny - yr.size)
-                for tz in range(ntz):
-                    z0, z1 = tz * TILE_Z, min(nz, (tz + 1) * TILE_Z) - 1
-                    zp = (z0 - R < zl.size) or (z1 + R >= nz - zr.size)
+                yp = (y0 - R < max(yl.size, 1)) or (y1 + R > ny - 1 - max(yr.size, 1))
+                for tz, (z0, zhi) in enumerate(zt):
+                    zp = (z0 - R < max(zl.size, 1)) or (zhi - 1 + R > nz - 1 - max(zr.size, 1))
                     flags[(tx * nty + ty) * ntz + tz] = 1 if (xp or yp or zp) else 0
         return (ntx, nty, ntz), flags
 
@@ -221,8 +221,13 @@ class DeviceBatch:
             z0 = np.where(tz == ntz - 1, zlast, tz * TILE_Z)
             return (pos[:, 0] // self.tile_x) * ntz + tz, (pos[:, 0] % self.tile_x) * TILE_Z + (z - z0)
         ty = self.tile_y
-        t = ((pos[:, 0] // self.tile_x) * nty + pos[:, 1] // ty) * ntz + pos[:, 2] // TILE_Z
-        local = ((pos[:, 0] % self.tile_x) * ty + pos[:, 1] % ty) * TILE_Z + pos[:, 2] % TILE_Z
+        nz = s.shape[2]
+        zlast = max(0, nz - TILE_Z)
+        z = pos[:, 2]
+        tz = np.where(z >= zlast, ntz - 1, z // TILE_Z)
+        z0 = np.where(tz == ntz - 1, zlast, tz * TILE_Z)
+        t = ((pos[:, 0] // self.tile_x) * nty + pos[:, 1] // ty) * ntz + tz
+        local = ((pos[:, 0] % self.tile_x) * ty + pos[:, 1] % ty) * TILE_Z + (z - z0)
         return t, local
 
     def _build_tables(self):
