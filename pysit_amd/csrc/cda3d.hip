@@ -39,7 +39,7 @@ constexpr int TX3 = 8;  // x planes per tile
 int tile_rows_3d(int axis) { return axis == 0 ? TX3 : TY3; }
 
 #ifndef PA3D_PREFETCH
-#define PA3D_PREFETCH 2
+#define PA3D_PREFETCH 1
 #endif
 
 template <typename T>
@@ -507,8 +507,13 @@ __device__ __forceinline__ void sample_group3(const pysit_amd_batch &B, int slot
     if (live && g == 0) S->trace[(long long)k * S->nrec + rd.y] = acc;
 }
 
+// at most 128 VGPRs (FP32): two 8-wave workgroups per CU
+#ifndef PA3D_WAVES_PER_EU
+#define PA3D_WAVES_PER_EU 4
+#endif
 template <typename T, int R, int MODE>
-__global__ __launch_bounds__(512) void k_step3d(const pysit_amd_batch B, const Step3DParams<T> P,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? PA3D_WAVES_PER_EU : 2)))
+void k_step3d(const pysit_amd_batch B, const Step3DParams<T> P,
                                                 int slot, int seg_base, int n_tile_blocks,
                                                 int n_rec_lanes, int rec_blocks)
 {
