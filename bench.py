@@ -52,8 +52,11 @@ def parse():
     p.add_argument("--tile-x", type=int, default=None)
     p.add_argument("--memory-budget-gb", type=float, default=None)
     p.add_argument("--profile-run", action="store_true", help="minimal run for rocprofv3 (no plain, no CPU)")
-    p.add_argument("--workload", choices=["c2", "c3"], default="c2",
-                   help="c2: the headline 16-shot gradient; c3: 64 shots of uneven aperture and length")
+    p.add_argument("--workload", choices=["c2", "c3", "c5"], default="c2",
+                   help="c2: the headline 16-shot gradient; c3: 64 shots of uneven aperture and length; "
+                        "c5: the multi-band LBFGS FWI loop (FP32, then FP64 for comparison)")
+    p.add_argument("--fwi-iterations", type=int, default=20, help="c5: LBFGS iterations per band")
+    p.add_argument("--fwi-bands", default="3,5,8", help="c5: Ricker peak frequencies (Hz), low to high")
     p.add_argument("--pmc-run", action="store_true",
                    help="short run for rocprofv3 --pmc: a 256 MiB elementwise calibration kernel, then one "
                         "gradient of the workload over --nt slots (default 40)")
@@ -212,6 +215,72 @@ def run_uneven(args):
         "host_setup_s": t_setup}), flush=True)
 
 
+def run_fwi(args):
+    """The 2D FWI loop: 601 x 201 nodes at dx = 15 m (641 x 241 padded), the
+    layered true model, its 300 m Gaussian smoothing as the start model, 32
+    shots with 601 receivers, nt = 2500 at the true model's CFL step, order
+    8; for each band (Ricker peaks 3, 5, 8 Hz) observed data from the true
+    model and --fwi-iterations LBFGS iterations (memory 10) of
+    TemporalLeastSquares with the backtracking line search.  Run in FP32
+    (timed: value = algorithmic grid-point updates of every forward, gradient
+    and line-search evaluation over the wall time of the loop, host work
+    included) and again in FP64 on the device, whose gradients match the FP64
+    oracle to 1e-10 in the GPU tests; prints one JSON line."""
+    import torch
+    from pysit_amd.workloads import fwi_2d, multiband_fwi, model_error
+    bands = tuple(float(f) for f in args.fwi_bands.split(","))
+    out = {}
+    for prec in ("single", "double"):
+        if prec == "double" and args.no_accuracy:
+            break
+        mesh, solver, true_model, start_model = fwi_2d(precision=prec)
+        print(f"[c5] {prec}: bands {bands}, {args.fwi_iterations} iterations each", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        model, reports = multiband_fwi(mesh, solver, true_model, start_model, bands=bands,
+                                       iterations=args.fwi_iterations, verbose=True, sync=torch.cuda.synchronize,
+                                       log=sys.stderr)
+        torch.cuda.synchronize()
+        out[prec] = {"model": model, "reports": reports, "seconds": time.perf_counter() - t0,
+                     "start_error": model_error(start_model, true_model), "true": true_model}
+        torch.cuda.empty_cache()
+    s32 = out["single"]
+    updates = sum(r["updates"] for r in s32["reports"])
+    value = updates / s32["seconds"] / 1e9
+
+    def band_summary(reports):
+        return [{"f0_hz": r["f0_hz"], "objective_first": r["objective"][0], "objective_final": r["objective"][-1],
+                 "misfit_reduction": r["misfit_reduction"], "model_error_start": r["model_error_start"],
+                 "model_error_final": r["model_error_final"], "evaluations": r["evaluations"],
+                 "gradients": r["gradients"], "seconds": r["seconds"]} for r in reports]
+    line = {
+        "metric": "C5 multi-band FWI loop Gpt-updates/s (forward, gradient and line-search evaluations, "
+                  "host work included)",
+        "value": value, "unit": "Gpt-updates/s", "n_gpus": 1, "steps": len(bands) * args.fwi_iterations,
+        "warmup": 0, "ms_per_step": s32["seconds"] * 1e3 / (len(bands) * args.fwi_iterations),
+        "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "2D FWI, 601x201 (641x241 padded), dx=15 m, order 8, PML 20, 32 shots, "
+                               "601 receivers, nt 2500 at the true model's CFL dt, layered + random field true "
+                               "model, 300 m smoothed start, LBFGS memory 10, backtracking line search",
+                   "bands_hz": list(bands), "iterations_per_band": args.fwi_iterations},
+        "fp32": {"seconds": s32["seconds"], "updates_gpt": updates / 1e9,
+                 "model_error_start": s32["start_error"],
+                 "model_error_final": model_error(s32["model"], s32["true"]),
+                 "bands": band_summary(s32["reports"])},
+    }
+    if "double" in out:
+        s64 = out["double"]
+        c32, c64 = np.asarray(s32["model"].C).ravel(), np.asarray(s64["model"].C).ravel()
+        o32 = np.concatenate([r["objective"] for r in s32["reports"]])
+        o64 = np.concatenate([r["objective"] for r in s64["reports"]])
+        n = min(len(o32), len(o64))
+        line["fp64"] = {"seconds": s64["seconds"], "model_error_final": model_error(s64["model"], s64["true"]),
+                        "bands": band_summary(s64["reports"])}
+        line["fp32_vs_fp64"] = {
+            "final_model_rel_l2": float(np.linalg.norm(c32 - c64) / np.linalg.norm(c64)),
+            "objective_history_max_rel": float(np.max(np.abs(o32[:n] - o64[:n]) / np.abs(o64[:n])))}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -224,6 +293,9 @@ def main():
     from pysit_amd.workloads import layered_2d
     if args.workload == "c3":
         run_uneven(args)
+        return
+    if args.workload == "c5":
+        run_fwi(args)
         return
     if args.pmc_run:
         # calibration: y = x + 1 over 2^26 floats reads and writes exactly 256 MiB

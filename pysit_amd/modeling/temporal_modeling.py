@@ -76,11 +76,30 @@ class TemporalModeling(object):
         return ShotGeometry(shape, s.kernel_pml(), s.model_window(), s.nsteps, src, amp, rec,
                             shot.receivers.receiver_count, adj=adj)
 
-    def batch(self, shots, imaging_period=1):
+    def batch(self, shots, imaging_period=1, reuse=False):
+        """The device batch of ``shots`` for the solver's current model.  With
+        ``reuse`` the last reusable batch is kept: the same shots on the same
+        time axis (the evaluations of an optimisation loop) reuse its tables,
+        state and stores and only take the new model.  Per-shot calls never
+        reuse, so device handles they return keep their own model and
+        history."""
+        import torch
         s = self.solver
-        return DeviceBatch([self.geometry(sh) for sh in shots], s.unpadded_model(), s.mesh.deltas, s.dt,
-                           s.spatial_accuracy_order, dtype=s.dtype, imaging_period=imaging_period,
-                           memory_budget=self.memory_budget, tile_x=self.tile_x, ic_window=self.ic_window)
+        key = (tuple(id(sh) for sh in shots), s.dt, s.nsteps, imaging_period, np.dtype(s.dtype).str,
+               s.spatial_accuracy_order, tuple(s.mesh.shape(include_bc=True, as_grid=True)))
+        cached = getattr(self, "_batch_cache", None)
+        if reuse and cached is not None and cached[0] == key:
+            b = cached[1]
+            b.model.copy_(torch.from_numpy(np.ascontiguousarray(s.unpadded_model(), dtype=b.dtype)))
+            return b
+        if reuse:
+            self._batch_cache = None
+        b = DeviceBatch([self.geometry(sh) for sh in shots], s.unpadded_model(), s.mesh.deltas, s.dt,
+                        s.spatial_accuracy_order, dtype=s.dtype, imaging_period=imaging_period,
+                        memory_budget=self.memory_budget, tile_x=self.tile_x, ic_window=self.ic_window)
+        if reuse:
+            self._batch_cache = (key, b, list(shots))
+        return b
 
     def _unpad_field(self, padded):
         mesh = self.solver.mesh
@@ -119,7 +138,7 @@ class TemporalModeling(object):
         With ``keep_dw`` the batch is kept in ``self.last_batch`` for
         :meth:`gradient_batch`-style reuse."""
         self.solver.model_parameters = m0
-        b = self.batch(shots, imaging_period)
+        b = self.batch(shots, imaging_period, reuse=True)
         traces = b.forward(keep_dw=keep_dw)
         self.last_batch = b if keep_dw else None
         return [t.cpu().numpy() for t in traces]
@@ -189,7 +208,7 @@ class TemporalModeling(object):
         results, list of operands)."""
         solver = self.solver
         solver.model_parameters = m0
-        b = self.batch(shots, imaging_period)
+        b = self.batch(shots, imaging_period, reuse=True)
         traces = b.forward(keep_dw=True)
         ops = [np.asarray(operands_fn(i, t.cpu().numpy()), dtype=np.float64) for i, t in enumerate(traces)]
         b.migrate(ops)

@@ -13,6 +13,8 @@ package's own API, seeded so that every run sees the same numbers:
 * ``uneven_2d`` -- shots of seeded random aperture (a window of the layered
   model's columns) and step count, as device geometries for one batch.
 * ``layered_3d`` -- the 3D analogue with a Gaussian low-velocity anomaly.
+* ``fwi_2d`` / ``multiband_fwi`` -- the 2D FWI case (layered true model,
+  smoothed start model) and its frequency-continuation LBFGS loop.
 """
 import numpy as np
 from scipy.ndimage import gaussian_filter
@@ -22,7 +24,7 @@ from .core import (RectangularDomain, PML, Dirichlet, CartesianMesh, RickerWavel
 from .solvers import ConstantDensityAcousticWave
 
 __all__ = ["horizontal_reflector_2d", "layered_velocity_2d", "layered_2d", "uneven_2d", "layered_velocity_3d",
-           "layered_3d", "pml_amplitude", "trange_for_steps"]
+           "layered_3d", "fwi_2d", "multiband_fwi", "model_error", "pml_amplitude", "trange_for_steps"]
 
 
 def pml_amplitude(vmax, width, reflection=1e-3):
@@ -179,3 +181,124 @@ def layered_3d(n=256, dx=12.5, n_shots=8, nt=2000, order=8, precision="single", 
     model = solver.ModelParameters(m, {"C": v.reshape(-1, 1)})
     solver.model_parameters = model
     return m, shots, solver, model
+
+
+def fwi_2d(nx=601, nz=201, dx=15.0, nt=2500, order=8, precision="single", pml_cells=20, seed=0,
+           start_sigma=300.0):
+    """The 2D FWI case: the layered model family on nx x nz nodes as the true
+    model, and the true model Gaussian-smoothed with ``start_sigma`` metres
+    (edges extended) as the start model.  The time range gives ``nt`` steps
+    at the CFL dt of the true model; every other model the solver takes
+    re-derives dt (and the step count) from its own maximum speed.
+
+    Returns (mesh, solver, true ModelParameters, start ModelParameters)."""
+    v = layered_velocity_2d(nx, nz, dx, seed=seed)
+    v0 = gaussian_filter(v, sigma=start_sigma / dx, mode="nearest")
+    width = pml_cells * dx
+    p = PML(width, pml_amplitude(v.max(), width))
+    d = RectangularDomain((0.0, (nx - 1) * dx, p, p), (0.0, (nz - 1) * dx, p, p))
+    m = CartesianMesh(d, nx, nz)
+    dt = (1.0 / 6.0) * dx / v.max()
+    solver = ConstantDensityAcousticWave(m, spatial_accuracy_order=order, trange=trange_for_steps(dt, nt),
+                                         kernel_implementation="hip", precision=precision)
+    true_model = solver.ModelParameters(m, {"C": v.reshape(-1, 1)})
+    start_model = solver.ModelParameters(m, {"C": v0.reshape(-1, 1)})
+    solver.model_parameters = true_model
+    assert solver.nsteps == nt, (solver.nsteps, nt)
+    return m, solver, true_model, start_model
+
+
+class _CountedObjective(object):
+    """TemporalLeastSquares that counts its evaluations, gradients and
+    algorithmic grid-point updates (forward: shots x padded nodes x steps at
+    the model's dt; gradient: twice that)."""
+
+    def __init__(self, solver, n_nodes, **kwargs):
+        from .objective_functions import TemporalLeastSquares
+        self.inner = TemporalLeastSquares(solver, **kwargs)
+        self.solver = solver
+        self.n_nodes = n_nodes
+        self.evaluations = self.gradients = 0
+        self.updates = 0.0
+
+    def use_parallel(self):
+        return self.inner.use_parallel()
+
+    @property
+    def parallel_wrap_shot(self):
+        return self.inner.parallel_wrap_shot
+
+    def evaluate(self, shots, m0, **kwargs):
+        v = self.inner.evaluate(shots, m0, **kwargs)
+        self.evaluations += 1
+        self.updates += len(shots) * self.n_nodes * self.solver.nsteps
+        return v
+
+    def compute_gradient(self, shots, m0, aux_info={}, **kwargs):
+        g = self.inner.compute_gradient(shots, m0, aux_info=aux_info, **kwargs)
+        self.gradients += 1
+        self.updates += 2.0 * len(shots) * self.n_nodes * self.solver.nsteps
+        return g
+
+
+def model_error(model, true_model):
+    """Relative L2 error of the wave speed over the unpadded domain."""
+    c, ct = np.asarray(model.C).ravel(), np.asarray(true_model.C).ravel()
+    return float(np.linalg.norm(c - ct) / np.linalg.norm(ct))
+
+
+def multiband_fwi(mesh, solver, true_model, start_model, bands=(3.0, 5.0, 8.0), iterations=20,
+                  memory_length=10, n_shots=32, depth=None, verbose=False, sync=None, log=None):
+    """Frequency-continuation FWI with LBFGS over TemporalLeastSquares.
+
+    For each Ricker peak frequency in ``bands`` (low to high): new shots with
+    that wavelet (``n_shots`` sources and one receiver per surface node at
+    ``depth``, equispaced acquisition), observed data generated in the true
+    model (the time path of generate_seismic_data, on the true model's time
+    axis), then ``iterations`` LBFGS iterations with ``memory_length`` pairs
+    and the backtracking line search, starting from the previous band's
+    result.  Per band this is the inversion pattern of
+    examples/Marmousi2D.py:18-69 (acquisition, generate_seismic_data,
+    TemporalLeastSquares, LBFGS, invalg(shots, initial_value, nsteps,
+    line_search='backtrack')); the trial models' dt changes with their maximum
+    speed, and the objective interpolates the observed data onto it
+    (objective_functions/temporal_least_squares.py:56).
+
+    ``sync`` (optional) is called before each band's timer is read, e.g. a
+    device synchronisation; ``log`` (a file) takes the LBFGS progress lines
+    when ``verbose``.  Returns (final model, one report dict per band:
+    frequency, objective history, misfit reduction, model error before and
+    after, evaluations, gradients, grid-point updates, seconds)."""
+    import time
+    from .modeling import generate_seismic_data
+    from .optimization import LBFGS
+    depth = 2 * mesh.x.delta if depth is None else depth
+    n_nodes = int(np.prod(mesh.shape(include_bc=True, as_grid=True)))
+    model = start_model
+    reports = []
+    for f0 in bands:
+        t0 = time.perf_counter()
+        shots = equispaced_acquisition(mesh, RickerWavelet(f0), sources=n_shots, source_depth=depth,
+                                       receivers="max", receiver_depth=depth)
+        generate_seismic_data(shots, solver, true_model)
+        obj = _CountedObjective(solver, n_nodes)
+        obj.updates += len(shots) * n_nodes * solver.nsteps
+        err0 = model_error(model, true_model)
+        invalg = LBFGS(obj, memory_length=memory_length)
+        if log is not None:
+            invalg.logfile = log
+        status = {"objective_frequency": 1, "gradient_length_frequency": 1, "alpha_frequency": 1}
+        model = invalg(shots, model, iterations, line_search="backtrack", status_configuration=status,
+                       verbose=verbose)
+        # objective of the final model (the history holds the iterates before each step)
+        f_final = obj.evaluate(shots, model)
+        if sync is not None:
+            sync()
+        hist = [float(v) for _, v in sorted(invalg.objective_history.items())]
+        reports.append({
+            "f0_hz": float(f0), "iterations": int(iterations), "objective": hist + [float(f_final)],
+            "misfit_reduction": float(f_final / hist[0]) if hist and hist[0] > 0 else None,
+            "model_error_start": err0, "model_error_final": model_error(model, true_model),
+            "evaluations": obj.evaluations, "gradients": obj.gradients, "updates": obj.updates,
+            "seconds": time.perf_counter() - t0})
+    return model, reports

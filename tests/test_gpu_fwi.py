@@ -1,0 +1,87 @@
+"""The FWI loop on the GPU (GPU tests): batch reuse across objective calls,
+and the multi-band LBFGS driver on a small layered case in FP32 and FP64."""
+import copy
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_fwi(precision):
+    from pysit_amd.workloads import fwi_2d
+    return fwi_2d(nx=121, nz=61, dx=15.0, nt=1200, precision=precision, start_sigma=150.0)
+
+
+def _shots(mesh, f0, n=6):
+    import pysit_amd as pa
+    return pa.equispaced_acquisition(mesh, pa.RickerWavelet(f0), sources=n, source_depth=30.0, receivers="max",
+                                     receiver_depth=30.0)
+
+
+def test_objective_reuses_its_batch_across_models(gpu):
+    """evaluate / compute_gradient at a new model on the same time axis reuse
+    the device batch; the results equal those of a fresh objective."""
+    import pysit_amd as pa
+    mesh, solver, true_model, start_model = _small_fwi("double")
+    shots = _shots(mesh, 8.0)
+    pa.generate_seismic_data(shots, solver, true_model)
+    obj = pa.TemporalLeastSquares(solver)
+    f0 = obj.evaluate(shots, start_model)
+    b0 = obj.modeling_tools._batch_cache[1]
+    # an interior perturbation leaves the maximum speed, so dt, unchanged
+    m1 = copy.deepcopy(start_model)
+    C = np.asarray(m1.C).reshape(121, 61).copy()
+    C[40:80, 20:40] *= 0.97
+    m1.C = C.reshape(-1, 1)
+    solver.model_parameters = m1
+    f1 = obj.evaluate(shots, m1)
+    assert obj.modeling_tools._batch_cache[1] is b0
+    aux = {"objective_value": (True, None)}
+    g1 = obj.compute_gradient(shots, m1, aux_info=aux)
+    assert obj.modeling_tools._batch_cache[1] is b0
+    fresh = pa.TemporalLeastSquares(solver)
+    f1_ref = fresh.evaluate(shots, m1)
+    g1_ref = fresh.compute_gradient(shots, m1)
+    assert f1 != f0
+    assert abs(f1 - f1_ref) <= 1e-13 * abs(f1_ref)
+    assert abs(aux["objective_value"][1] - f1_ref) <= 1e-12 * abs(f1_ref)
+    d = np.asarray(g1.data) - np.asarray(g1_ref.data)
+    assert np.linalg.norm(d) <= 1e-13 * np.linalg.norm(np.asarray(g1_ref.data))
+
+
+@pytest.fixture(scope="module")
+def fwi_runs():
+    """Two bands (5 and 8 Hz) of 3 LBFGS iterations each, FP32 and FP64."""
+    from pysit_amd.workloads import multiband_fwi
+    out = {}
+    for prec in ("single", "double"):
+        mesh, solver, true_model, start_model = _small_fwi(prec)
+        model, reports = multiband_fwi(mesh, solver, true_model, start_model, bands=(5.0, 8.0), iterations=3,
+                                       n_shots=6, depth=30.0)
+        out[prec] = (model, reports)
+    return out
+
+
+def test_multiband_fwi_reduces_misfit_and_model_error(gpu, fwi_runs):
+    for prec, (model, reports) in fwi_runs.items():
+        for r in reports:
+            print(prec, r["f0_hz"], r["objective"], r["model_error_start"], r["model_error_final"], flush=True)
+            assert r["gradients"] == 3 and r["evaluations"] >= 4
+            assert r["objective"][-1] < 0.5 * r["objective"][0]
+            assert all(np.isfinite(r["objective"]))
+        assert reports[-1]["model_error_final"] < reports[0]["model_error_start"]
+
+
+def test_multiband_fwi_fp32_follows_fp64(gpu, fwi_runs):
+    """The FP32 loop takes the FP64 loop's path: the first objective value
+    agrees to the FP32 trace error, the final models closely."""
+    m32, r32 = fwi_runs["single"]
+    m64, r64 = fwi_runs["double"]
+    for a, b in zip(r32, r64):
+        print(a["objective"], b["objective"], flush=True)
+        assert abs(a["objective"][0] - b["objective"][0]) <= 1e-3 * b["objective"][0]
+    c32, c64 = np.asarray(m32.C).ravel(), np.asarray(m64.C).ravel()
+    err = np.linalg.norm(c32 - c64) / np.linalg.norm(c64)
+    print("final model FP32 vs FP64", err, flush=True)
+    assert err < 1e-3
