@@ -207,6 +207,43 @@ int pysit_amd_batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int 
                         int direction, int seg_base, const int *active_tiles,
                         const int *active_recs, void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* Plans: the batch time loop as replayed hipGraphs.                         */
+/*                                                                           */
+/* The reference steps its time loop from Python, one step call per time    */
+/* level (temporal_modeling.py:120-142 forward, :313-350 adjoint).  A plan   */
+/* is the opaque handle that runs pysit_amd_batch_run's loop as graphs: it   */
+/* owns a device step cursor, a capture stream and a cache of instantiated   */
+/* graphs.  A graph holds `graph_steps` consecutive step launches (and the   */
+/* FP32 window flushes among them) that read their slot from the cursor,    */
+/* then a node that advances the cursor, so one graph serves every chunk of */
+/* a run with the same batch contents, mode, direction and active counts.   */
+/* Chunks whose active tile or receiver counts change inside them, and the  */
+/* remainder of a run, are launched step by step.  Results are identical to  */
+/* pysit_amd_batch_run's.                                                    */
+/*   graph_steps: steps per graph (>= 1; with the FP32 window accumulator a */
+/*                multiple of ic_window, else that run goes step by step)   */
+/*   max_graphs:  graphs kept (least recently used evicted), >= 1           */
+/* A plan belongs to the device current at creation; its calls must not    */
+/* overlap (one stream at a time).  Graphs capture the batch by value: a    */
+/* batch whose tables or buffers change is a new cache entry.               */
+/* ------------------------------------------------------------------------ */
+typedef struct pysit_amd_plan pysit_amd_plan;
+
+int pysit_amd_plan_create(int graph_steps, int max_graphs, pysit_amd_plan **out);
+
+/* pysit_amd_batch_run through the plan (same arguments and checks). */
+int pysit_amd_plan_run(pysit_amd_plan *plan, const pysit_amd_batch *b, int mode, int slot_begin,
+                       int slot_end, int direction, int seg_base, const int *active_tiles,
+                       const int *active_recs, void *stream);
+
+/* stats[0] graph launches, [1] graph captures, [2] steps launched one by
+ * one, [3] graphs cached now */
+int pysit_amd_plan_stats(const pysit_amd_plan *plan, long long *stats);
+
+/* Waits for the device, then frees the plan's graphs, cursor and stream. */
+int pysit_amd_plan_destroy(pysit_amd_plan *plan);
+
 /* Fold (ModelPerturbationBase.add_padding, pysit/solvers/model_parameter.py:
  * 537-593) of one padded FP64 image onto its unpadded grid: the PML slabs are
  * summed onto the first/last physical plane of each axis (x, then y, then z),

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpysit_amd.so")
-SOURCES = ["api.hip", "cda2d.hip", "cda3d.hip"]
+SOURCES = ["api.hip", "cda2d.hip", "cda3d.hip", "plan.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

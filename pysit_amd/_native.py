@@ -6,6 +6,7 @@ for the solver path.
 """
 import ctypes
 import os
+import weakref
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpysit_amd.so")
@@ -68,6 +69,10 @@ SIGNATURES = {
     "pysit_amd_cda_step3d_f64": _STEP3D,
     "pysit_amd_batch_run": [ctypes.POINTER(Batch), _i, _i, _i, _i, _i, _vp, _vp, _vp],
     "pysit_amd_fold": [_i] * 10 + [_vp, _d, _vp, _i, _i, _i, _vp],
+    "pysit_amd_plan_create": [_i, _i, ctypes.POINTER(_vp)],
+    "pysit_amd_plan_run": [_vp, ctypes.POINTER(Batch), _i, _i, _i, _i, _i, _vp, _vp, _vp],
+    "pysit_amd_plan_stats": [_vp, _vp],
+    "pysit_amd_plan_destroy": [_vp],
 }
 
 
@@ -114,3 +119,34 @@ def call(name, *args):
 def exported_symbols():
     """Entry points declared in include/pysit_amd.h."""
     return ["pysit_amd_last_error", "pysit_amd_version"] + list(SIGNATURES)
+
+
+class Plan(object):
+    """A pysit_amd_plan: the batch time loop as replayed hipGraphs of
+    ``graph_steps`` steps (at most ``max_graphs`` kept).  Freed with the
+    object (or by ``close``)."""
+
+    def __init__(self, graph_steps, max_graphs=16):
+        h = _vp()
+        call("pysit_amd_plan_create", int(graph_steps), int(max_graphs), ctypes.byref(h))
+        self.graph_steps = int(graph_steps)
+        self._h = h
+        self._fin = weakref.finalize(self, _plan_destroy, h.value)
+
+    def run(self, b, mode, j0, j1, direction, seg_base, active_tiles, active_recs, stream):
+        call("pysit_amd_plan_run", self._h, ctypes.byref(b), mode, j0, j1, direction, seg_base, active_tiles,
+             active_recs, stream)
+
+    def stats(self):
+        """{'graph_launches', 'captures', 'direct_steps', 'cached'}"""
+        out = (ctypes.c_longlong * 4)()
+        call("pysit_amd_plan_stats", self._h, ctypes.cast(out, _vp))
+        return dict(zip(("graph_launches", "captures", "direct_steps", "cached"), list(out)))
+
+    def close(self):
+        self._fin()
+
+
+def _plan_destroy(handle):
+    if handle and _lib is not None:
+        _lib.pysit_amd_plan_destroy(handle)

@@ -10,9 +10,11 @@ void set_error(const std::string &msg) { g_err = msg; }
 void clear_error() { g_err.clear(); }
 
 template <typename T>
-int run2d(const pysit_amd_batch &, int, int, int, int, int, const int *, const int *, hipStream_t);
+int run2d(const pysit_amd_batch &, int, int, int, int, int, const int *, const int *, pysit_amd_plan *,
+          hipStream_t);
 template <typename T>
-int run3d(const pysit_amd_batch &, int, int, int, int, int, const int *, const int *, hipStream_t);
+int run3d(const pysit_amd_batch &, int, int, int, int, int, const int *, const int *, pysit_amd_plan *,
+          hipStream_t);
 
 int tile_rows_2d(int dtype);
 int tile_rows_3d(int axis);
@@ -178,11 +180,15 @@ int pysit_amd_cda_step3d_f64(int order, const double *km1_u, const double *k_phi
                                 kp1_phiy, kp1_phiz, kp1_psi, kp1_u, (hipStream_t)stream);
 }
 
-int pysit_amd_batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int slot_end,
-                        int direction, int seg_base, const int *active_tiles,
-                        const int *active_recs, void *stream)
+}  // extern "C"
+
+namespace pa {
+
+// pysit_amd_batch_run's checks and dispatch, with an optional plan
+static int batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int slot_end, int direction,
+                     int seg_base, const int *active_tiles, const int *active_recs, pysit_amd_plan *plan,
+                     void *stream)
 {
-    clear_error();
     PA_CHECK(b != nullptr, "null batch");
     const pysit_amd_batch &B = *b;
     PA_CHECK(B.dim == 2 || B.dim == 3, "dim must be 2 or 3");
@@ -219,13 +225,41 @@ int pysit_amd_batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int 
     hipStream_t st = (hipStream_t)stream;
     if (B.dim == 2)
         return B.dtype ? run2d<double>(B, mode, slot_begin, slot_end, direction, seg_base,
-                                       active_tiles, active_recs, st)
+                                       active_tiles, active_recs, plan, st)
                        : run2d<float>(B, mode, slot_begin, slot_end, direction, seg_base,
-                                      active_tiles, active_recs, st);
+                                      active_tiles, active_recs, plan, st);
     return B.dtype ? run3d<double>(B, mode, slot_begin, slot_end, direction, seg_base,
-                                   active_tiles, active_recs, st)
+                                   active_tiles, active_recs, plan, st)
                    : run3d<float>(B, mode, slot_begin, slot_end, direction, seg_base,
-                                  active_tiles, active_recs, st);
+                                  active_tiles, active_recs, plan, st);
+}
+
+int plan_device(const pysit_amd_plan *plan);
+
+}  // namespace pa
+
+extern "C" {
+
+int pysit_amd_batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int slot_end,
+                        int direction, int seg_base, const int *active_tiles,
+                        const int *active_recs, void *stream)
+{
+    clear_error();
+    return batch_run(b, mode, slot_begin, slot_end, direction, seg_base, active_tiles, active_recs, nullptr,
+                     stream);
+}
+
+int pysit_amd_plan_run(pysit_amd_plan *plan, const pysit_amd_batch *b, int mode, int slot_begin,
+                       int slot_end, int direction, int seg_base, const int *active_tiles,
+                       const int *active_recs, void *stream)
+{
+    clear_error();
+    PA_CHECK(plan != nullptr, "null plan");
+    int dev = -1;
+    PA_HIP(hipGetDevice(&dev));
+    PA_CHECK(dev == plan_device(plan), "the plan belongs to another device");
+    return batch_run(b, mode, slot_begin, slot_end, direction, seg_base, active_tiles, active_recs, plan,
+                     stream);
 }
 
 int pysit_amd_fold(int dim, int nx, int ny, int nz, int lx, int rx, int ly, int ry, int lz, int rz,

@@ -581,13 +581,19 @@ __device__ __forceinline__ void sample_group(const pysit_amd_batch &B, int slot,
 #ifndef PA2D_WAVES_PER_EU
 #define PA2D_WAVES_PER_EU 1
 #endif
+// With a step cursor (graph launches) the slot is cursor[0] + slot and the
+// dWaveOp store base is cursor[1]; the host checks the slot range.
 template <typename T, int R, int MODE, bool SINGLE>
 __global__ __launch_bounds__(256, PA2D_WAVES_PER_EU) void k_step2d(const pysit_amd_batch B, const Step2DParams<T> P,
                                                 int slot, int seg_base, int n_tile_waves,
-                                                int n_rec_lanes, int rec_blocks)
+                                                int n_rec_lanes, int rec_blocks, const int *cursor)
 {
     __shared__ T lds_f[4][Tile2D<T>::TX * 64];
     __shared__ T lds_z[4][4 * zrow_len<R>()];
+    if (cursor) {
+        slot += cursor[0];
+        seg_base = cursor[1];
+    }
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int b = blockIdx.x;
@@ -609,7 +615,7 @@ __global__ __launch_bounds__(256, PA2D_WAVES_PER_EU) void k_step2d(const pysit_a
 
 template <typename T, int R, int MODE>
 static int launch2d(const pysit_amd_batch &B, const Step2DParams<T> &P, int slot, int seg_base,
-                    int n_tiles, int n_recs, hipStream_t st)
+                    int n_tiles, int n_recs, hipStream_t st, const int *cursor = nullptr)
 {
     const int recs = is_adjoint(MODE) ? 0 : n_recs;
     // 16 receivers per block, block count rounded to whole XCD rounds
@@ -619,14 +625,14 @@ static int launch2d(const pysit_amd_batch &B, const Step2DParams<T> &P, int slot
     if (P.single) {
         if constexpr (MODE == 0) {
             hipLaunchKernelGGL((k_step2d<T, R, MODE, true>), dim3(rec_blocks + tile_blocks), dim3(256), 0, st,
-                               B, P, slot, seg_base, n_tiles, recs, rec_blocks);
+                               B, P, slot, seg_base, n_tiles, recs, rec_blocks, cursor);
         } else {
             set_error("the single-shot step only propagates");
             return PYSIT_AMD_EINVAL;
         }
     } else {
         hipLaunchKernelGGL((k_step2d<T, R, MODE, false>), dim3(rec_blocks + tile_blocks), dim3(256), 0, st,
-                           B, P, slot, seg_base, n_tiles, recs, rec_blocks);
+                           B, P, slot, seg_base, n_tiles, recs, rec_blocks, cursor);
     }
     PA_HIP(hipGetLastError());
     return PYSIT_AMD_OK;
@@ -634,13 +640,13 @@ static int launch2d(const pysit_amd_batch &B, const Step2DParams<T> &P, int slot
 
 template <typename T, int MODE>
 static int launch2d_r(int r, const pysit_amd_batch &B, const Step2DParams<T> &P, int slot,
-                      int seg_base, int n_tiles, int n_recs, hipStream_t st)
+                      int seg_base, int n_tiles, int n_recs, hipStream_t st, const int *cursor = nullptr)
 {
     switch (r) {
-    case 1: return launch2d<T, 1, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st);
-    case 2: return launch2d<T, 2, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st);
-    case 3: return launch2d<T, 3, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st);
-    case 4: return launch2d<T, 4, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st);
+    case 1: return launch2d<T, 1, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
+    case 2: return launch2d<T, 2, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
+    case 3: return launch2d<T, 3, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
+    case 4: return launch2d<T, 4, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
     }
     set_error("order must be 2, 4, 6 or 8");
     return PYSIT_AMD_EINVAL;
@@ -667,8 +673,34 @@ static int ic_flush(const pysit_amd_batch &B, hipStream_t st)
 }
 
 template <typename T>
+struct Sink2D final : StepSink {
+    const pysit_amd_batch &B;
+    const Step2DParams<T> &P;
+    int r, mode;
+    bool window;
+    Sink2D(const pysit_amd_batch &B_, const Step2DParams<T> &P_, int r_, int mode_, bool window_)
+        : B(B_), P(P_), r(r_), mode(mode_), window(window_) {}
+    int step(int slot, int seg_base, int nt, int nr, const int *cursor, hipStream_t st) override
+    {
+        switch (mode) {
+        case 0: return launch2d_r<T, 0>(r, B, P, slot, seg_base, nt, nr, st, cursor);
+        case 1: return launch2d_r<T, 1>(r, B, P, slot, seg_base, nt, nr, st, cursor);
+        case 3: return launch2d_r<T, 3>(r, B, P, slot, seg_base, nt, nr, st, cursor);
+        default: break;
+        }
+        if (!window) return launch2d_r<T, 2>(r, B, P, slot, seg_base, nt, 0, st, cursor);
+        if constexpr (std::is_same<T, float>::value)
+            return launch2d_r<T, 4>(r, B, P, slot, seg_base, nt, 0, st, cursor);
+        set_error("the FP32 window accumulator needs an FP32 batch");
+        return PYSIT_AMD_EINVAL;
+    }
+    int flush(hipStream_t st) override { return ic_flush(B, st); }
+};
+
+template <typename T>
 int run2d(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int direction,
-          int seg_base, const int *active_tiles, const int *active_recs, hipStream_t st)
+          int seg_base, const int *active_tiles, const int *active_recs, pysit_amd_plan *plan,
+          hipStream_t st)
 {
     PA_CHECK(B.tile_x == Tile2D<T>::TX, "2D tiles are " + std::to_string(Tile2D<T>::TX) +
                                             " rows along x for this dtype (tile_x)");
@@ -702,30 +734,16 @@ int run2d(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int 
     }
     PA_CHECK(B.total_n * 8 <= kMaxRsrcBytes, "batch too large for one launch (total_n)");
     const bool window = mode == 2 && B.ic32 != nullptr;
-    const int n = slot_end - slot_begin;
-    for (int i = 0; i < n; ++i) {
-        const int j = direction > 0 ? slot_begin + i : slot_end - 1 - i;
-        const int nt = active_tiles ? active_tiles[j] : B.n_tiles;
-        const int nr = active_recs ? active_recs[j] : B.n_rec;
-        int rc;
-        if (mode == 0) rc = launch2d_r<T, 0>(r, B, P, j, seg_base, nt, nr, st);
-        else if (mode == 1) rc = launch2d_r<T, 1>(r, B, P, j, seg_base, nt, nr, st);
-        else if (mode == 3) rc = launch2d_r<T, 3>(r, B, P, j, seg_base, nt, nr, st);
-        else if (!window) rc = launch2d_r<T, 2>(r, B, P, j, seg_base, nt, 0, st);
-        else {
-            if constexpr (std::is_same<T, float>::value) rc = launch2d_r<T, 4>(r, B, P, j, seg_base, nt, 0, st);
-            else rc = PYSIT_AMD_EINVAL;
-            if (!rc && ((i + 1) % B.ic_window == 0 || i + 1 == n)) rc = ic_flush(B, st);
-        }
-        if (rc) return rc;
-    }
-    return PYSIT_AMD_OK;
+    Sink2D<T> S(B, P, r, mode, window);
+    const LoopSpec L{mode, slot_begin, slot_end, direction, seg_base, active_tiles, active_recs,
+                     window ? B.ic_window : 0};
+    return time_loop(plan, B, L, S, st);
 }
 
 template int run2d<float>(const pysit_amd_batch &, int, int, int, int, int, const int *,
-                          const int *, hipStream_t);
+                          const int *, pysit_amd_plan *, hipStream_t);
 template int run2d<double>(const pysit_amd_batch &, int, int, int, int, int, const int *,
-                           const int *, hipStream_t);
+                           const int *, pysit_amd_plan *, hipStream_t);
 
 // One step with the reference contract (dense rhs, caller-owned outputs).
 template <typename T>

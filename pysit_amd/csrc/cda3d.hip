@@ -515,9 +515,14 @@ template <typename T, int R, int MODE>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? PA3D_WAVES_PER_EU : 2)))
 void k_step3d(const pysit_amd_batch B, const Step3DParams<T> P,
                                                 int slot, int seg_base, int n_tile_blocks,
-                                                int n_rec_lanes, int rec_blocks)
+                                                int n_rec_lanes, int rec_blocks, const int *cursor)
 {
     __shared__ Lds3D<T, R> L;
+    // with a step cursor (graph launches): slot offset from cursor[0], store base cursor[1]
+    if (cursor) {
+        slot += cursor[0];
+        seg_base = cursor[1];
+    }
     const int b = blockIdx.x;
     if (MODE != 2 && b < rec_blocks) {
         const int rec = b * (TY3 * 4) + (threadIdx.x >> 4);
@@ -536,35 +541,55 @@ void k_step3d(const pysit_amd_batch B, const Step3DParams<T> P,
 
 template <typename T, int R, int MODE>
 static int launch3d(const pysit_amd_batch &B, const Step3DParams<T> &P, int slot, int seg_base,
-                    int n_tiles, int n_recs, hipStream_t st)
+                    int n_tiles, int n_recs, hipStream_t st, const int *cursor = nullptr)
 {
     const int recs = (MODE != 2) ? n_recs : 0;
     const int rec_blocks = recs ? (recs + TY3 * 4 - 1) / (TY3 * 4) : 0;
     const int blocks = n_tiles + rec_blocks;
     if (blocks == 0) return PYSIT_AMD_OK;
     hipLaunchKernelGGL((k_step3d<T, R, MODE>), dim3(blocks), dim3(TY3 * 64), 0, st, B, P, slot,
-                       seg_base, n_tiles, recs, rec_blocks);
+                       seg_base, n_tiles, recs, rec_blocks, cursor);
     PA_HIP(hipGetLastError());
     return PYSIT_AMD_OK;
 }
 
 template <typename T, int MODE>
 static int launch3d_r(int r, const pysit_amd_batch &B, const Step3DParams<T> &P, int slot,
-                      int seg_base, int n_tiles, int n_recs, hipStream_t st)
+                      int seg_base, int n_tiles, int n_recs, hipStream_t st, const int *cursor = nullptr)
 {
     switch (r) {
-    case 1: return launch3d<T, 1, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st);
-    case 2: return launch3d<T, 2, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st);
-    case 3: return launch3d<T, 3, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st);
-    case 4: return launch3d<T, 4, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st);
+    case 1: return launch3d<T, 1, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
+    case 2: return launch3d<T, 2, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
+    case 3: return launch3d<T, 3, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
+    case 4: return launch3d<T, 4, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
     }
     set_error("order must be 2, 4, 6 or 8");
     return PYSIT_AMD_EINVAL;
 }
 
 template <typename T>
+struct Sink3D final : StepSink {
+    const pysit_amd_batch &B;
+    const Step3DParams<T> &P;
+    int r, mode;
+    Sink3D(const pysit_amd_batch &B_, const Step3DParams<T> &P_, int r_, int mode_)
+        : B(B_), P(P_), r(r_), mode(mode_) {}
+    int step(int slot, int seg_base, int nt, int nr, const int *cursor, hipStream_t st) override
+    {
+        switch (mode) {
+        case 0: return launch3d_r<T, 0>(r, B, P, slot, seg_base, nt, nr, st, cursor);
+        case 1: return launch3d_r<T, 1>(r, B, P, slot, seg_base, nt, nr, st, cursor);
+        case 2: return launch3d_r<T, 2>(r, B, P, slot, seg_base, nt, 0, st, cursor);
+        default: return launch3d_r<T, 3>(r, B, P, slot, seg_base, nt, nr, st, cursor);
+        }
+    }
+    int flush(hipStream_t) override { return PYSIT_AMD_OK; }
+};
+
+template <typename T>
 int run3d(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int direction,
-          int seg_base, const int *active_tiles, const int *active_recs, hipStream_t st)
+          int seg_base, const int *active_tiles, const int *active_recs, pysit_amd_plan *plan,
+          hipStream_t st)
 {
     PA_CHECK(B.tile_x == TX3 && B.tile_y == TY3, "3D tiles are 8 x 8 (tile_x, tile_y)");
     Step3DParams<T> P{};
@@ -575,24 +600,15 @@ int run3d(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int 
     P.tc = time_coef<T>(B.dt);
     P.rhs = nullptr;
     P.single = 0;
-    for (int i = 0; i < slot_end - slot_begin; ++i) {
-        const int j = direction > 0 ? slot_begin + i : slot_end - 1 - i;
-        const int nt = active_tiles ? active_tiles[j] : B.n_tiles;
-        const int nr = active_recs ? active_recs[j] : B.n_rec;
-        int rc;
-        if (mode == 0) rc = launch3d_r<T, 0>(r, B, P, j, seg_base, nt, nr, st);
-        else if (mode == 1) rc = launch3d_r<T, 1>(r, B, P, j, seg_base, nt, nr, st);
-        else if (mode == 2) rc = launch3d_r<T, 2>(r, B, P, j, seg_base, nt, 0, st);
-        else rc = launch3d_r<T, 3>(r, B, P, j, seg_base, nt, nr, st);
-        if (rc) return rc;
-    }
-    return PYSIT_AMD_OK;
+    Sink3D<T> S(B, P, r, mode);
+    const LoopSpec L{mode, slot_begin, slot_end, direction, seg_base, active_tiles, active_recs, 0};
+    return time_loop(plan, B, L, S, st);
 }
 
 template int run3d<float>(const pysit_amd_batch &, int, int, int, int, int, const int *,
-                          const int *, hipStream_t);
+                          const int *, pysit_amd_plan *, hipStream_t);
 template int run3d<double>(const pysit_amd_batch &, int, int, int, int, int, const int *,
-                           const int *, hipStream_t);
+                           const int *, pysit_amd_plan *, hipStream_t);
 
 template <typename T>
 int step3d_entry(int order, const T *km1_u, const T *k_phix, const T *k_phiy, const T *k_phiz,

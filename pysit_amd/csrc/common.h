@@ -166,4 +166,24 @@ __device__ __forceinline__ T sigma_mesh(const T *l, int nl, const T *r, int nr, 
     return T(0);
 }
 
+// ---- batch time loop -------------------------------------------------------
+// One dimension's step launcher as the loop driver sees it: `step` launches
+// the step of `slot` (with a cursor: the offset from the cursor's slot),
+// `flush` the FP32 window accumulator's flush.
+struct StepSink {
+    virtual int step(int slot, int seg_base, int n_tiles, int n_recs, const int *cursor, hipStream_t st) = 0;
+    virtual int flush(hipStream_t st) = 0;
+    virtual ~StepSink() {}
+};
+
+struct LoopSpec {
+    int mode, slot_begin, slot_end, direction, seg_base;
+    const int *active_tiles, *active_recs;
+    int window;  // flush every `window` steps of the call and after its last (0: never)
+};
+
+// Runs the steps of `L` on `st`: one launch per step, or, with a plan, in
+// graph chunks of the plan's step count (see plan.hip).
+int time_loop(pysit_amd_plan *plan, const pysit_amd_batch &B, const LoopSpec &L, StepSink &S, hipStream_t st);
+
 }  // namespace pa

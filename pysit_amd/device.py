@@ -109,11 +109,14 @@ def _as_entries(e, dim):
     return pos, np.asarray(chan, dtype=np.int64).reshape(-1), np.asarray(w, dtype=np.float64).reshape(-1)
 
 
+GRAPH_MAX_NODES = 1 << 20
+
+
 class DeviceBatch:
     """Tables and state of a shot batch on one GPU."""
 
     def __init__(self, shots, model, deltas, dt, order, dtype=np.float32, tile_x=None,
-                 imaging_period=1, memory_budget=None, ic_window=None, tile_order="shot"):
+                 imaging_period=1, memory_budget=None, ic_window=None, tile_order="shot", graph_steps=None):
         if not shots:
             raise ValueError("empty shot batch")
         self.dev = _dev()
@@ -151,6 +154,16 @@ class DeviceBatch:
         if tile_order not in ("shot", "interleave"):
             raise ValueError("tile_order must be 'shot' or 'interleave'")
         self.tile_order = tile_order
+        # time loop as replayed hipGraphs of `graph_steps` steps (0: one launch
+        # per step); the plan is made on the first run.  Default: graphs for
+        # batches of fewer than GRAPH_MAX_NODES nodes, whose steps are short
+        # enough for launch gaps to show; larger batches launch step by step
+        # (a graph step reads its slot from device memory first, which costs
+        # them 1-3 %).
+        self._graph_steps_arg = graph_steps
+        if graph_steps is not None and int(graph_steps) < 0:
+            raise ValueError("graph_steps must be >= 0")
+        self.plan = None
 
         if isinstance(model, torch.Tensor):
             self.model = model.detach().to(device=self.dev, dtype=self.tdtype).contiguous()
@@ -170,6 +183,8 @@ class DeviceBatch:
         self.n_shots = len(self.shots)
         self._build_tables()
         self._alloc_state()
+        gs = self._graph_steps_arg
+        self.graph_steps = int(gs) if gs is not None else (32 if self.total_n < GRAPH_MAX_NODES else 0)
 
     # ------------------------------------------------------------------ tables
     def _tiles_of(self, s):
@@ -456,8 +471,13 @@ class DeviceBatch:
         if self.timer is not None:
             ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
             ev[0].record()
-        N.call("pysit_amd_batch_run", ctypes.byref(b), mode, j0, j1, direction, seg_base,
-               at.ctypes.data_as(ctypes.c_void_p), ar.ctypes.data_as(ctypes.c_void_p), _stream())
+        atp, arp = at.ctypes.data_as(ctypes.c_void_p), ar.ctypes.data_as(ctypes.c_void_p)
+        if self.graph_steps:
+            if self.plan is None:
+                self.plan = N.Plan(self.graph_steps)
+            self.plan.run(b, mode, j0, j1, direction, seg_base, atp, arp, _stream())
+        else:
+            N.call("pysit_amd_batch_run", ctypes.byref(b), mode, j0, j1, direction, seg_base, atp, arp, _stream())
         if ev is not None:
             ev[1].record()
             self.timer.setdefault(mode, []).append((ev[0], ev[1], j1 - j0))

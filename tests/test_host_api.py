@@ -175,3 +175,22 @@ def test_solver_path_fails_loudly_without_gpu():
     with pytest.raises(Exception) as e:
         tm.forward_model(shots[0], model, return_parameters=["simdata"])
     assert "GPU" in str(e.value) or "ROCm" in str(e.value)
+
+
+def test_plan_entry_points_check_arguments_without_a_device():
+    """pysit_amd_plan_create rejects bad sizes before touching a device; the
+    other plan entry points reject a null plan."""
+    from pysit_amd import _native as N
+    if not os.path.exists(N.LIB_PATH):
+        pytest.skip("library not built (python -m pysit_amd._build)")
+    L = N.lib()
+    h = ctypes.c_void_p()
+    for gs, mg, what in ((0, 4, "graph_steps"), (5000, 4, "graph_steps"), (32, 0, "max_graphs")):
+        assert L.pysit_amd_plan_create(gs, mg, ctypes.byref(h)) != 0
+        assert what in L.pysit_amd_last_error().decode()
+        assert h.value is None
+    assert L.pysit_amd_plan_run(None, None, 0, 0, 1, 1, 0, None, None, None) != 0
+    assert "null plan" in L.pysit_amd_last_error().decode()
+    stats = (ctypes.c_longlong * 4)()
+    assert L.pysit_amd_plan_stats(None, ctypes.cast(stats, ctypes.c_void_p)) != 0
+    assert L.pysit_amd_plan_destroy(None) == 0
