@@ -33,10 +33,15 @@ const char *pysit_amd_last_error(void);
 /* Library version string. */
 const char *pysit_amd_version(void);
 
-/* Tile geometry the library was built with: rows along x (axis 0) or y
- * (axis 1, 3D) of one tile, for dim 2 or 3 and dtype 0 (f32) or 1 (f64).
- * pysit_amd_batch.tile_x / tile_y must carry these values. */
+/* Default tile geometry: rows along x (axis 0) or y (axis 1, 3D) of one
+ * tile, for dim 2 or 3 and dtype 0 (f32) or 1 (f64).  A batch's tile_x /
+ * tile_y carry these values or others pysit_amd_tile_rows_supported accepts
+ * (2D: 8 or 4 rows along x; 3D: 8 x 8). */
 int pysit_amd_tile_rows(int dim, int dtype, int axis);
+
+/* 1 if a batch of this dim and dtype may use tile_x rows along x, else 0
+ * (negative on bad arguments). */
+int pysit_amd_tile_rows_supported(int dim, int dtype, int tile_x);
 
 /* ABI self-check for bindings: returns 0 when the caller's sizes of
  * pysit_amd_shot and pysit_amd_batch match the library's, else -1. */

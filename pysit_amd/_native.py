@@ -63,6 +63,7 @@ _STEP3D = [_i] + [_vp] * 8 + [_vp, _i] * 6 + [_d] * 4 + [_i] * 3 + [_vp] * 5 + [
 SIGNATURES = {
     "pysit_amd_abi_check": [_i, _i],
     "pysit_amd_tile_rows": [_i, _i, _i],
+    "pysit_amd_tile_rows_supported": [_i, _i, _i],
     "pysit_amd_cda_step2d_f32": _STEP2D,
     "pysit_amd_cda_step2d_f64": _STEP2D,
     "pysit_amd_cda_step3d_f32": _STEP3D,
@@ -105,6 +106,14 @@ def tile_rows(dim, dtype_code, axis=0):
     if n <= 0:
         raise NativeError(lib().pysit_amd_last_error().decode())
     return n
+
+
+def tile_rows_supported(dim, dtype_code, tile_x):
+    """Whether a batch may use ``tile_x`` rows along x."""
+    n = lib().pysit_amd_tile_rows_supported(dim, dtype_code, int(tile_x))
+    if n < 0:
+        raise NativeError(lib().pysit_amd_last_error().decode())
+    return n == 1
 
 
 def call(name, *args):

@@ -17,6 +17,7 @@ int run3d(const pysit_amd_batch &, int, int, int, int, int, const int *, const i
           hipStream_t);
 
 int tile_rows_2d(int dtype);
+bool tile_rows_2d_supported(int tx);
 int tile_rows_3d(int axis);
 
 template <typename T>
@@ -105,6 +106,17 @@ int pysit_amd_tile_rows(int dim, int dtype, int axis)
     }
     if (dim == 2) return axis == 0 ? pa::tile_rows_2d(dtype) : 1;
     return pa::tile_rows_3d(axis);
+}
+
+int pysit_amd_tile_rows_supported(int dim, int dtype, int tile_x)
+{
+    clear_error();
+    if ((dim != 2 && dim != 3) || (dtype != 0 && dtype != 1)) {
+        set_error("tile_rows_supported: dim must be 2 or 3, dtype 0 or 1");
+        return PYSIT_AMD_EINVAL;
+    }
+    if (dim == 2) return pa::tile_rows_2d_supported(tile_x) ? 1 : 0;
+    return tile_x == pa::tile_rows_3d(0) ? 1 : 0;
 }
 
 int pysit_amd_abi_check(int shot_size, int batch_size)

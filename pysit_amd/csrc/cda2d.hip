@@ -8,7 +8,7 @@
 //
 // Mapping to CDNA: one wavefront owns a tile of 64 consecutive z (the
 // contiguous axis: every row access is one coalesced 256 B transaction) by
-// TX = 8 rows along x.  The z tiles of a shot start at
+// TX = 8 or 4 rows along x.  The z tiles of a shot start at
 // 0, 64, 128, ... except the last, which ends on the last node; the node
 // ranges the tiles own are disjoint (a tile lane that another tile owns only
 // loads), so the PML band at the far z side costs one PML tile, not two.
@@ -50,17 +50,11 @@
 
 namespace pa {
 
-// Tile rows (compile-time tuning knobs; the host reads them back through
-// pysit_amd_tile_rows).
-#ifndef PA2D_TX_F32
-#define PA2D_TX_F32 8
-#endif
-#ifndef PA2D_TX_F64
-#define PA2D_TX_F64 8
-#endif
-template <typename T> struct Tile2D;
-template <> struct Tile2D<float> { static constexpr int TX = PA2D_TX_F32; };
-template <> struct Tile2D<double> { static constexpr int TX = PA2D_TX_F64; };
+// Tile rows along x: every 2D kernel is built for kTileRows2D[0] (the
+// default) and kTileRows2D[1] rows; a batch picks one (tile_x).  Short tiles
+// give more wavefronts and shorter marches (small grids), tall ones re-read
+// fewer halo rows.
+constexpr int kTileRows2D[2] = {8, 4};
 
 // state buffers of one step: u_k, u_{k-1} (overwritten by u_{k+1}),
 // Phix_k, Phix_{k+1}, Phiz_k, Phiz_{k+1}
@@ -167,10 +161,47 @@ template <int R> constexpr int zrow_len() { return 2 * R + 128; }
 // order; wave_sync keeps the compiler from moving the reads above the writes,
 // and rows alternate between buffers so the next row's writes cannot move
 // above this row's reads either.
+#ifndef PA2D_ZDPP
+#define PA2D_ZDPP 0
+#endif
+// wavefront rotations by one lane (DPP): rol, lane l takes lane l+1 (mod 64);
+// ror, lane l takes lane l-1
+template <int CTRL>
+__device__ __forceinline__ float wrot(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double wrot(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+constexpr int kDppRol1 = 0x134, kDppRor1 = 0x13C;
+
 template <typename T, int R, bool PACKED>
 __device__ __forceinline__ void z_taps(const Ctx2D<T, R, PACKED> &c, int buf, T v, T h, T (&zp)[R + 1],
                                        T (&zm)[R + 1])
 {
+#if PA2D_ZDPP
+    // lane l's neighbour at z+i is lane l+i, or past the last lane the right
+    // halo value that lane l+i-64 holds in h; at z-i likewise lane l-i, or the
+    // left halo that lane l-i+64 holds
+    T a = v, b = v, ha = h, hb = h;
+#pragma unroll
+    for (int i = 1; i <= R; ++i) {
+        a = wrot<kDppRol1>(a);
+        ha = wrot<kDppRol1>(ha);
+        b = wrot<kDppRor1>(b);
+        hb = wrot<kDppRor1>(hb);
+        zp[i] = c.lane + i < 64 ? a : ha;
+        zm[i] = c.lane - i >= 0 ? b : hb;
+    }
+    (void)buf;
+    return;
+#endif
     T *row = c.zbuf + buf * zrow_len<R>();
     row[c.lane + R] = v;
     row[c.hslot] = h;
@@ -214,13 +245,29 @@ __device__ __forceinline__ void ic_store(const C &c, int o, int xr, typename Acc
 #ifndef PA2D_PREFETCH
 #define PA2D_PREFETCH 2
 #endif
+// the dWaveOp history streams through (written once, read back thousands of
+// steps later): non-temporal accesses keep it from evicting the fields
+#ifndef PA2D_DW_NT
+#define PA2D_DW_NT 1
+#endif
+template <typename T>
+__device__ __forceinline__ T dw_ld(Rsrc r, int voff, int soff)
+{
+    if constexpr (PA2D_DW_NT) return buf_ld_nt<T>(r, voff, soff);
+    else return buf_ld<T>(r, voff, soff);
+}
+template <typename T>
+__device__ __forceinline__ void dw_st(Rsrc r, int voff, int soff, T v)
+{
+    if constexpr (PA2D_DW_NT) buf_st_nt(r, voff, soff, v);
+    else buf_st(r, voff, soff, v);
+}
 
 // Plain tiles: no PML node within r of an owned node, no edge node, every
 // row of the march inside the grid.
-template <typename T, int R, int MODE, bool PACKED>
+template <typename T, int TX, int R, int MODE, bool PACKED>
 __device__ __forceinline__ void march_int(const Ctx2D<T, R, PACKED> &c, const Step2DParams<T> &P, int x0)
 {
-    constexpr int TX = Tile2D<T>::TX;
     constexpr int D = PA2D_PREFETCH;
     constexpr int NU = TX + 2 * R;  // u rows x0-R .. x0+TX+R-1
     const int rb = c.rowb;
@@ -237,7 +284,7 @@ __device__ __forceinline__ void march_int(const Ctx2D<T, R, PACKED> &c, const St
         const int mr = clampi(x - c.mx_off, c.mx_lo, c.mx_hi) * c.mrowb;                           \
         cv[r] = buf_ld<T>(c.model, mvo, mr);                                                        \
         fv[r] = c.finj[r * c.fstride];                                                              \
-        dv[r] = (MODE >= 2) ? buf_ld<T>(c.dw, ovd + xr, c.shb) : T(0);                              \
+        dv[r] = (MODE >= 2) ? dw_ld<T>(c.dw, ovd + xr, c.shb) : T(0);                              \
         mv[r] = (MODE == 3) ? buf_ld<T>(c.m1, mvo, mr) : T(0);                                      \
         if (is_adjoint(MODE)) icv[r] = ic_load<T, MODE>(c, ovi, xr);                                \
     } while (0)
@@ -272,7 +319,7 @@ __device__ __forceinline__ void march_int(const Ctx2D<T, R, PACKED> &c, const St
         const T up = fma(P.tc.dt2, c2 * src, T(2) * v - um[j]);
         if (is_adjoint(MODE)) ic_store<T, MODE>(c, ovi, xr, icv[j], v, dv[j]);
         buf_st(c.s.rs(F_UO), ov + xr, so_uo, up);
-        if (MODE == 1) buf_st(c.dw, ovd + xr, c.shb, c2 * src);
+        if (MODE == 1) dw_st<T>(c.dw, ovd + xr, c.shb, c2 * src);
         if (c.has_fst) buf_st(c.fst, ovf + xr, c.shb, v);
     }
 #undef PA_LD_U
@@ -280,10 +327,9 @@ __device__ __forceinline__ void march_int(const Ctx2D<T, R, PACKED> &c, const St
 }
 
 // PML tiles: the full update, edges, rows beyond the grid.
-template <typename T, int R, int MODE, bool PACKED>
+template <typename T, int TX, int R, int MODE, bool PACKED>
 __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const Step2DParams<T> &P, int x0)
 {
-    constexpr int TX = Tile2D<T>::TX;
     constexpr int D = PA2D_PREFETCH;
     constexpr int NU = TX + 2 * R;
     const int rb = c.rowb, nx = c.nx;
@@ -318,7 +364,7 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
         T f = PACKED ? T(0) : buf_ld<T>(c.rhs, ovr, 0); /* dense rhs: reference-contract step */   \
         f += c.finj[r * c.fstride];                                                                 \
         fv[r] = f;                                                                                  \
-        dv[r] = (MODE >= 2) ? buf_ld<T>(c.dw, own ? ovd + xr : kOOB, c.shb) : T(0);                 \
+        dv[r] = (MODE >= 2) ? dw_ld<T>(c.dw, own ? ovd + xr : kOOB, c.shb) : T(0);                 \
         mv[r] = (MODE == 3) ? buf_ld<T>(c.m1, mvo, mr) : T(0);                                      \
         if (is_adjoint(MODE)) icv[r] = ic_load<T, MODE>(c, own ? ovi : kOOB, xr);                  \
         pv[r] = buf_ld<T>(c.s.rs(F_PZ), own ? (xp ? zv : zpv) + xr : kOOB, so_pz);                  \
@@ -401,20 +447,19 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
         const int op = phi_store ? ovr : kOOB;
         buf_st(c.s.rs(F_PXO), op, c.s.so(F_PXO), pxn);
         buf_st(c.s.rs(F_PZO), op, c.s.so(F_PZO), pzn);
-        if (MODE == 1) buf_st(c.dw, own ? ovd + xr : kOOB, c.shb, dw);
+        if (MODE == 1) dw_st<T>(c.dw, own ? ovd + xr : kOOB, c.shb, dw);
         if (c.has_fst) buf_st(c.fst, own ? ovf + xr : kOOB, c.shb, v);
     }
 #undef PA_LD_U
 #undef PA_LD_ROW
 }
 
-template <typename T, int R, int MODE, bool SINGLE>
+template <typename T, int TX, int R, int MODE, bool SINGLE>
 __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DParams<T> &P,
                                        const pysit_amd_shot &S, int tile, int full, int tloc,
                                        int slot, int seg_base, int lane, T *lds_f, T *zbuf)
 {
     constexpr bool PACKED = !SINGLE;
-    constexpr int TX = Tile2D<T>::TX;
     constexpr int SZ = sizeof(T);
     const int k = slot - S.kshift;
     if (k < 0 || k >= S.nt) return;
@@ -549,8 +594,8 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
         wave_sync();
     }
 
-    if (full) march_full<T, R, MODE, PACKED>(c, P, x0);
-    else march_int<T, R, MODE, PACKED>(c, P, x0);
+    if (full) march_full<T, TX, R, MODE, PACKED>(c, P, x0);
+    else march_int<T, TX, R, MODE, PACKED>(c, P, x0);
 }
 
 // Receiver sampling: 16 lanes per receiver (4 receivers per wavefront); lane
@@ -583,12 +628,12 @@ __device__ __forceinline__ void sample_group(const pysit_amd_batch &B, int slot,
 #endif
 // With a step cursor (graph launches) the slot is cursor[0] + slot and the
 // dWaveOp store base is cursor[1]; the host checks the slot range.
-template <typename T, int R, int MODE, bool SINGLE>
+template <typename T, int TX, int R, int MODE, bool SINGLE>
 __global__ __launch_bounds__(256, PA2D_WAVES_PER_EU) void k_step2d(const pysit_amd_batch B, const Step2DParams<T> P,
                                                 int slot, int seg_base, int n_tile_waves,
                                                 int n_rec_lanes, int rec_blocks, const int *cursor)
 {
-    __shared__ T lds_f[4][Tile2D<T>::TX * 64];
+    __shared__ T lds_f[4][TX * 64];
     __shared__ T lds_z[4][4 * zrow_len<R>()];
     if (cursor) {
         slot += cursor[0];
@@ -606,16 +651,17 @@ __global__ __launch_bounds__(256, PA2D_WAVES_PER_EU) void k_step2d(const pysit_a
     const int gw = xcd_block(b - rec_blocks, gridDim.x - rec_blocks) * 4 + w;
     if (gw >= n_tile_waves) return;
     if (SINGLE) {
-        tile2d<T, R, MODE, SINGLE>(B, P, P.shot0, gw, 1, gw, slot, seg_base, lane, lds_f[w], lds_z[w]);
+        tile2d<T, TX, R, MODE, SINGLE>(B, P, P.shot0, gw, 1, gw, slot, seg_base, lane, lds_f[w], lds_z[w]);
     } else {
         const int4 td = reinterpret_cast<const int4 *>(B.tiles)[gw];
-        tile2d<T, R, MODE, SINGLE>(B, P, B.shots[td.x], gw, td.z & 1, td.y, slot, seg_base, lane, lds_f[w], lds_z[w]);
+        tile2d<T, TX, R, MODE, SINGLE>(B, P, B.shots[td.x], gw, td.z & 1, td.y, slot, seg_base, lane, lds_f[w],
+                                       lds_z[w]);
     }
 }
 
-template <typename T, int R, int MODE>
+template <typename T, int TX, int R, int MODE>
 static int launch2d(const pysit_amd_batch &B, const Step2DParams<T> &P, int slot, int seg_base,
-                    int n_tiles, int n_recs, hipStream_t st, const int *cursor = nullptr)
+                    int n_tiles, int n_recs, hipStream_t st, const int *cursor)
 {
     const int recs = is_adjoint(MODE) ? 0 : n_recs;
     // 16 receivers per block, block count rounded to whole XCD rounds
@@ -624,35 +670,48 @@ static int launch2d(const pysit_amd_batch &B, const Step2DParams<T> &P, int slot
     if (rec_blocks + tile_blocks == 0) return PYSIT_AMD_OK;
     if (P.single) {
         if constexpr (MODE == 0) {
-            hipLaunchKernelGGL((k_step2d<T, R, MODE, true>), dim3(rec_blocks + tile_blocks), dim3(256), 0, st,
+            hipLaunchKernelGGL((k_step2d<T, TX, R, MODE, true>), dim3(rec_blocks + tile_blocks), dim3(256), 0, st,
                                B, P, slot, seg_base, n_tiles, recs, rec_blocks, cursor);
         } else {
             set_error("the single-shot step only propagates");
             return PYSIT_AMD_EINVAL;
         }
     } else {
-        hipLaunchKernelGGL((k_step2d<T, R, MODE, false>), dim3(rec_blocks + tile_blocks), dim3(256), 0, st,
+        hipLaunchKernelGGL((k_step2d<T, TX, R, MODE, false>), dim3(rec_blocks + tile_blocks), dim3(256), 0, st,
                            B, P, slot, seg_base, n_tiles, recs, rec_blocks, cursor);
     }
     PA_HIP(hipGetLastError());
     return PYSIT_AMD_OK;
 }
 
-template <typename T, int MODE>
-static int launch2d_r(int r, const pysit_amd_batch &B, const Step2DParams<T> &P, int slot,
-                      int seg_base, int n_tiles, int n_recs, hipStream_t st, const int *cursor = nullptr)
+template <typename T, int TX, int MODE>
+static int launch2d_tx(int r, const pysit_amd_batch &B, const Step2DParams<T> &P, int slot, int seg_base,
+                       int n_tiles, int n_recs, hipStream_t st, const int *cursor)
 {
     switch (r) {
-    case 1: return launch2d<T, 1, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
-    case 2: return launch2d<T, 2, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
-    case 3: return launch2d<T, 3, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
-    case 4: return launch2d<T, 4, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
+    case 1: return launch2d<T, TX, 1, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
+    case 2: return launch2d<T, TX, 2, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
+    case 3: return launch2d<T, TX, 3, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
+    case 4: return launch2d<T, TX, 4, MODE>(B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
     }
     set_error("order must be 2, 4, 6 or 8");
     return PYSIT_AMD_EINVAL;
 }
 
-int tile_rows_2d(int dtype) { return dtype ? Tile2D<double>::TX : Tile2D<float>::TX; }
+template <typename T, int MODE>
+static int launch2d_r(int r, const pysit_amd_batch &B, const Step2DParams<T> &P, int slot,
+                      int seg_base, int n_tiles, int n_recs, hipStream_t st, const int *cursor = nullptr)
+{
+    switch (B.tile_x) {
+    case kTileRows2D[0]: return launch2d_tx<T, kTileRows2D[0], MODE>(r, B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
+    case kTileRows2D[1]: return launch2d_tx<T, kTileRows2D[1], MODE>(r, B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
+    }
+    set_error("order must be 2, 4, 6 or 8");
+    return PYSIT_AMD_EINVAL;
+}
+
+int tile_rows_2d(int) { return kTileRows2D[0]; }
+bool tile_rows_2d_supported(int tx) { return tx == kTileRows2D[0] || tx == kTileRows2D[1]; }
 
 // ic += ic32; ic32 = 0 (the FP32 window accumulator's flush)
 __global__ void k_ic_flush(float *__restrict__ ic32, double *__restrict__ ic, long long n)
@@ -702,8 +761,9 @@ int run2d(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int 
           int seg_base, const int *active_tiles, const int *active_recs, pysit_amd_plan *plan,
           hipStream_t st)
 {
-    PA_CHECK(B.tile_x == Tile2D<T>::TX, "2D tiles are " + std::to_string(Tile2D<T>::TX) +
-                                            " rows along x for this dtype (tile_x)");
+    PA_CHECK(B.tile_x == kTileRows2D[0] || B.tile_x == kTileRows2D[1],
+             "2D tiles are " + std::to_string(kTileRows2D[0]) + " or " + std::to_string(kTileRows2D[1]) +
+                 " rows along x (tile_x)");
     Step2DParams<T> P{};
     const int r = B.order / 2;
     P.cx = axis_coef<T>(r, B.dx);
@@ -772,7 +832,7 @@ int step2d_entry(int order, const T *km1_u, const T *k_phix, const T *k_phiz, co
     B.dtype = sizeof(T) == 8;
     B.dim = 2;
     B.order = order;
-    B.tile_x = Tile2D<T>::TX;
+    B.tile_x = kTileRows2D[0];
     B.dt = dt;
     B.dx = dx;
     B.dz = dz;

@@ -146,6 +146,27 @@ __device__ __forceinline__ void buf_st(Rsrc r, int voff, int soff, double v)
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, soff, 0);
 }
 
+// Streaming (non-temporal) forms for data that is not reused soon, such as
+// the dWaveOp history: the caches keep the fields that are
+template <typename T> __device__ __forceinline__ T buf_ld_nt(Rsrc r, int voff, int soff);
+template <> __device__ __forceinline__ float buf_ld_nt<float>(Rsrc r, int voff, int soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 2));
+}
+template <> __device__ __forceinline__ double buf_ld_nt<double>(Rsrc r, int voff, int soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 2));
+}
+__device__ __forceinline__ void buf_st_nt(Rsrc r, int voff, int soff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, 2);
+}
+__device__ __forceinline__ void buf_st_nt(Rsrc r, int voff, int soff, double v)
+{
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, soff, 2);
+}
+
 // Kernel-side PML profile value at node i of an axis of n nodes (reference
 // indexing: left array by node, right array offset by one towards the edge;
 // the edge node itself is never updated, callers pass i <= n-2).

@@ -134,11 +134,15 @@ class DeviceBatch:
         self.deltas = tuple(float(d) for d in deltas)
         self.dt = float(dt)
         self.imaging_period = int(imaging_period)
-        # the kernels' register blocking fixes the tile rows per dimension and dtype
-        need = N.tile_rows(self.dim, 1 if self.dtype == np.float64 else 0, 0)
-        if tile_x and int(tile_x) != need:
-            raise ValueError(f"{self.dim}D tiles are {need} rows along x for {self.dtype}")
-        self.tile_x = need
+        # tile rows along x: the library's default, or another height its
+        # kernels are built for (2D: 8 or 4)
+        code = 1 if self.dtype == np.float64 else 0
+        if tile_x:
+            if not N.tile_rows_supported(self.dim, code, int(tile_x)):
+                raise ValueError(f"{self.dim}D {self.dtype} batches do not support tile_x={tile_x}")
+            self.tile_x = int(tile_x)
+        else:
+            self.tile_x = N.tile_rows(self.dim, code, 0)
         self.tile_y = N.tile_rows(self.dim, 1 if self.dtype == np.float64 else 0, 1) if self.dim == 3 else 1
         self.memory_budget = memory_budget
         # FP32 2D adjoints sum the imaging condition in FP32 over windows of
@@ -677,6 +681,48 @@ class DeviceBatch:
                 self._run(fb, 1, j0, j1, 1, j0, record=False)
                 self._run(qb, 2, j0, j1, -1, j0, record=False)
         return self.ic
+
+    def probe_step_time(self, slots=16, reps=2):
+        """Seconds per slot of the forward step storing dWaveOp (mode 1 with
+        sources) plus the adjoint imaging step, without receiver sampling and
+        adjoint sources, over the last ``slots`` slots (where every shot is
+        active), from rest; the best of ``reps`` timed passes after one
+        warm-up.  Works on scratch state: the batch's state, traces and
+        stores are untouched."""
+        S = max(1, min(int(slots), self.max_nt))
+        j0 = self.max_nt - S
+        state = self._new_state()
+        qstate = self._new_state()
+        dw = torch.empty((S, self.total_n), dtype=self.tdtype, device=self.dev)
+        ic = torch.zeros(self.total_n, dtype=torch.float64, device=self.dev)
+        fb = self._batch(state, False, dw=dw)
+        fb.n_rec = 0
+        qb = self._batch(qstate, False, ic=ic, dw=dw)
+        qb.n_rec = 0
+        qb.inj_ptr = qb.inj_node = qb.inj_nptr = qb.inj_chan = qb.inj_w = None
+        ic32 = None
+        if self.ic_window:
+            ic32 = torch.zeros(self.total_n, dtype=torch.float32, device=self.dev)
+            qb.ic32 = ic32.data_ptr()
+            qb.ic_window = self.ic_window
+        timer, self.timer = self.timer, None
+        best = None
+        try:
+            for rep in range(int(reps) + 1):
+                ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+                torch.cuda.synchronize()
+                ev[0].record()
+                self._run(fb, 1, j0, self.max_nt, 1, j0, record=False)
+                self._run(qb, 2, j0, self.max_nt, -1, j0, record=False)
+                ev[1].record()
+                torch.cuda.synchronize()
+                if rep:
+                    t = ev[0].elapsed_time(ev[1]) * 1e-3 / S
+                    best = t if best is None else min(best, t)
+        finally:
+            self.timer = timer
+        del state, qstate, dw, ic, ic32
+        return best
 
     def image(self, si_caller, scale=1.0):
         """Padded FP64 image of one shot (caller order), a view of the accumulator."""

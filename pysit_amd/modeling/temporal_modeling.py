@@ -56,7 +56,7 @@ class TemporalModeling(object):
     solver_type = "time"
     modeling_type = "time"
 
-    def __init__(self, solver, memory_budget=None, tile_x=None, ic_window=None):
+    def __init__(self, solver, memory_budget=None, tile_x=None, ic_window=None, autotune=True):
         if self.solver_type != solver.supports["equation_dynamics"]:
             raise TypeError(f"Argument 'solver' type {solver.supports['equation_dynamics']} does not match "
                             f"modeling solver type {self.solver_type}.")
@@ -64,6 +64,9 @@ class TemporalModeling(object):
         self.memory_budget = memory_budget
         self.tile_x = tile_x
         self.ic_window = ic_window  # None: the batch default (FP32 2D: 32 slots)
+        # launch configuration (tile rows, tile order, graphs) picked by timing
+        # the candidates once per batch signature, unless tile_x is given
+        self.autotune = bool(autotune)
 
     # ---------------------------------------------------------------- plumbing
     def geometry(self, shot):
@@ -94,12 +97,23 @@ class TemporalModeling(object):
             return b
         if reuse:
             self._batch_cache = None
-        b = DeviceBatch([self.geometry(sh) for sh in shots], s.unpadded_model(), s.mesh.deltas, s.dt,
-                        s.spatial_accuracy_order, dtype=s.dtype, imaging_period=imaging_period,
-                        memory_budget=self.memory_budget, tile_x=self.tile_x, ic_window=self.ic_window)
+        b = self.make_batch([self.geometry(sh) for sh in shots], s.unpadded_model(), imaging_period)
         if reuse:
             self._batch_cache = (key, b, list(shots))
         return b
+
+    def make_batch(self, geometries, model, imaging_period=1):
+        """A DeviceBatch of ``geometries`` for ``model`` (unpadded wave speed,
+        host array or device tensor) on the solver's time axis, autotuned
+        unless disabled or tile_x is fixed."""
+        s = self.solver
+        kw = dict(dtype=s.dtype, imaging_period=imaging_period, memory_budget=self.memory_budget,
+                  ic_window=self.ic_window)
+        if self.autotune and not self.tile_x:
+            from ..autotune import tuned_batch
+            return tuned_batch(geometries, model, s.mesh.deltas, s.dt, s.spatial_accuracy_order, **kw)
+        return DeviceBatch(geometries, model, s.mesh.deltas, s.dt, s.spatial_accuracy_order, tile_x=self.tile_x,
+                           **kw)
 
     def _unpad_field(self, padded):
         mesh = self.solver.mesh

@@ -30,11 +30,12 @@ class ShotProblem(object):
     per model change by ``TemporalModeling.geometry``)."""
 
     def __init__(self, solver, shots, imaging_period=1, memory_budget=None, tile_x=None, segment=None,
-                 ic_window=None):
+                 ic_window=None, autotune=True):
         from .modeling.temporal_modeling import TemporalModeling
         self.solver = solver
         self.shots = shots
-        self.tm = TemporalModeling(solver, memory_budget=memory_budget, tile_x=tile_x, ic_window=ic_window)
+        self.tm = TemporalModeling(solver, memory_budget=memory_budget, tile_x=tile_x, ic_window=ic_window,
+                                   autotune=autotune)
         self.imaging_period = imaging_period
         self.segment = segment
         self.batch = None
@@ -55,10 +56,7 @@ class ShotProblem(object):
         C = (m.detach() ** -0.5).to(torch.float64)
         if self.batch is None or self._key != key:
             self.batch = None
-            self.batch = DeviceBatch([self.tm.geometry(sh) for sh in self.shots], C, s.mesh.deltas, s.dt,
-                                     s.spatial_accuracy_order, dtype=s.dtype, imaging_period=self.imaging_period,
-                                     memory_budget=self.tm.memory_budget, tile_x=self.tm.tile_x,
-                                     ic_window=self.tm.ic_window)
+            self.batch = self.tm.make_batch([self.tm.geometry(sh) for sh in self.shots], C, self.imaging_period)
             self._key = key
         else:
             self.batch.model.copy_(C.reshape(self.batch.model.shape))
