@@ -377,6 +377,10 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
     __builtin_amdgcn_sched_barrier(0);  // the prologue's loads go out before any row work
     const T szk = c.szk2[0] + c.szk2[1], szm = c.szm2[0] + c.szm2[1];
     const T vsig = c.vsig2[0] + c.vsig2[1];
+    // per-lane factors of the rows outside the x PML
+    const T dtszk = P.tc.dt * szk;
+    const T fac1z = P.tc.two_dt2 / (T(2) + dtszk);
+    const T dsz = szm - szk;
 #pragma unroll
     for (int j = 0; j < TX; ++j) {
         if (j + 2 * R + D < NU) PA_LD_U(j + 2 * R + D);
@@ -419,22 +423,37 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
         const T sxmj = lane_bcast(vsig, TX + j);
         const T dt = P.tc.dt;
         const T pxc = px[R + j], pzc = pv[j];
-        const bool pml = sx != T(0) || szk != T(0);
-        // damped update (PML node) and plain update, selected per lane
-        const T pxd = (pxc - dt * sx * pxc) + dt * (szk - sx) * dux;
-        const T pzd = (pzc - dt * szk * pzc) + dt * (sx - szk) * duz;
-        const T ss = sx + szk;
-        const T fac1 = P.tc.two_dt2 / (T(2) + dt * ss);
-        const T fac2 = c2 * src - (u_m - T(2) * v) * P.tc.inv_dt2 + ss * u_m * P.tc.inv_2dt - (sx * szk) * v;
-        const T upd = fac1 * fac2;
         const T upi = fma(P.tc.dt2, c2 * src, T(2) * v - u_m);
-        T pxn = pml ? pxd : pxc;
-        T pzn = pml ? pzd : pzc;
-        T up = pml ? upd : upi;
+        T pxn, pzn, up, dw;
+        bool pml;
+        if (sx == T(0) && sxmj == T(0) && x > 0 && x < nx - 1) {
+            // a row outside the x PML (wave-uniform): the damped update below
+            // with sx = 0 and the per-lane division hoisted; the same values
+            // (zero terms dropped: at most the sign of a zero differs)
+            pml = szk != T(0);
+            const T pxd = pxc + dtszk * dux;
+            const T pzd = (pzc - dtszk * pzc) + (-dtszk) * duz;
+            const T fac2 = (c2 * src - (u_m - T(2) * v) * P.tc.inv_dt2) + szk * u_m * P.tc.inv_2dt;
+            pxn = pml ? pxd : pxc;
+            pzn = pml ? pzd : pzc;
+            up = pml ? fac1z * fac2 : upi;
+            dw = c2 * src + dsz * (up - u_m) * P.tc.inv_2dt;
+        } else {
+            pml = sx != T(0) || szk != T(0);
+            // damped update (PML node) and plain update, selected per lane
+            const T pxd = (pxc - dt * sx * pxc) + dt * (szk - sx) * dux;
+            const T pzd = (pzc - dt * szk * pzc) + dt * (sx - szk) * duz;
+            const T ss = sx + szk;
+            const T fac1 = P.tc.two_dt2 / (T(2) + dt * ss);
+            const T fac2 = c2 * src - (u_m - T(2) * v) * P.tc.inv_dt2 + ss * u_m * P.tc.inv_2dt - (sx * szk) * v;
+            pxn = pml ? pxd : pxc;
+            pzn = pml ? pzd : pzc;
+            up = pml ? fac1 * fac2 : upi;
+            // dWaveOp with the mesh profile: the step identity gives
+            // (u+ - 2u + u-)/dt^2 = c^2 src - sum(sk)(u+ - u-)/(2dt) - prod(sk) u
+            dw = c2 * src + ((sxmj + szm) - (sx + szk)) * (up - u_m) * P.tc.inv_2dt + (sxmj * szm - sx * szk) * v;
+        }
         bool phi_store = pml || c.general_phi;
-        // dWaveOp with the mesh profile: the step identity gives
-        // (u+ - 2u + u-)/dt^2 = c^2 src - sum(sk)(u+ - u-)/(2dt) - prod(sk) u
-        T dw = c2 * src + ((sxmj + szm) - (sx + szk)) * (up - u_m) * P.tc.inv_2dt + (sxmj * szm - sx * szk) * v;
         if (edge) {
             up = T(0);
             pxn = T(0);
@@ -456,7 +475,7 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
 
 template <typename T, int TX, int R, int MODE, bool SINGLE>
 __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DParams<T> &P,
-                                       const pysit_amd_shot &S, int tile, int full, int tloc,
+                                       const pysit_amd_shot &S, int tile, int full, int tloc, int tx,
                                        int slot, int seg_base, int lane, T *lds_f, T *zbuf)
 {
     constexpr bool PACKED = !SINGLE;
@@ -469,7 +488,7 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
     c.nz = nz;
     // z tiles: 0, 64, ... and a last one ending on node nz-1; owned ranges disjoint
     const int ntz = S.ntz;
-    const int tz = tloc % ntz, tx = tloc / ntz;
+    const int tz = tloc - tx * ntz;
     const int zlast = nz > 64 ? nz - 64 : 0;
     const int z0 = (tz == ntz - 1) ? zlast : tz * 64;
     const int own_hi = (tz == ntz - 1) ? nz : min(z0 + 64, zlast);
@@ -529,26 +548,9 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
     c.rhs = make_rsrc(c.has_rhs ? (const void *)P.rhs : vbase, c.has_rhs ? (long long)nx * nz * SZ : 0);
     c.zbuf = zbuf;
     c.hslot = lane < R ? R + 64 + lane : (lane >= 64 - R ? lane - (64 - R) : 2 * R + 64 + lane);
-    // PML profiles as masked buffer loads (no branches, so the loads stay
-    // ahead of the march in the wait-count order): lane z profile, and the x
-    // profile of the tile's rows, one lane per value (lane r < TX kernel
-    // form, lane TX + r mesh form), read back per row with readlane; rows
-    // past the grid repeat the last row
-    static_assert(2 * TX <= 64, "the x profile of a tile must fit one wavefront");
-    {
-        const Rsrc rzl = make_rsrc(S.nzl ? S.zl : vbase, (long long)S.nzl * SZ);
-        const Rsrc rzr = make_rsrc(S.nzr ? S.zr : vbase, (long long)S.nzr * SZ);
-        const Rsrc rxl = make_rsrc(S.nxl ? S.xl : vbase, (long long)S.nxl * SZ);
-        const Rsrc rxr = make_rsrc(S.nxr ? S.xr : vbase, (long long)S.nxr * SZ);
-        const bool zf = full && zin;
-        sigma_lane<T>(rzl, S.nzl, rzr, S.nzr, zf && z <= nz - 2 ? z : -1, nz, true, c.szk2);
-        sigma_lane<T>(rzl, S.nzl, rzr, S.nzr, zf ? z : -1, nz, false, c.szm2);
-        const int xs = min(x0 + (lane < TX ? lane : lane - TX), nx - 1);
-        const int xi = (full && lane < 2 * TX) ? (lane < TX ? max(0, min(xs, nx - 2)) : xs) : -1;
-        sigma_lane<T>(rxl, S.nxl, rxr, S.nxr, xi, nx, lane < TX, c.vsig2);
-    }
-    c.img = is_adjoint(MODE) && (k % B.imaging_period == 0) && B.dw;
-    const bool store = (MODE == 1) && (k % B.imaging_period == 0) && B.dw;
+    const bool imaged = B.imaging_period == 1 || k % B.imaging_period == 0;
+    c.img = is_adjoint(MODE) && imaged && B.dw;
+    const bool store = (MODE == 1) && imaged && B.dw;
     const long long slot_el = (long long)(slot - seg_base) * B.total_n;
     const long long slot_bytes = B.total_n * SZ;
     const bool dwon = ((c.img || MODE == 3) && B.dw) || store;
@@ -594,8 +596,27 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
         wave_sync();
     }
 
-    if (full) march_full<T, TX, R, MODE, PACKED>(c, P, x0);
-    else march_int<T, TX, R, MODE, PACKED>(c, P, x0);
+    if (full) {
+        // PML profiles as masked buffer loads (no branches, so the loads stay
+        // ahead of the march in the wait-count order): lane z profile, and the
+        // x profile of the tile's rows, one lane per value (lane r < TX kernel
+        // form, lane TX + r mesh form), read back per row with readlane; rows
+        // past the grid repeat the last row.  Plain tiles need none.
+        static_assert(2 * TX <= 64, "the x profile of a tile must fit one wavefront");
+        const void *vbase = B.u[0];
+        const Rsrc rzl = make_rsrc(S.nzl ? S.zl : vbase, (long long)S.nzl * SZ);
+        const Rsrc rzr = make_rsrc(S.nzr ? S.zr : vbase, (long long)S.nzr * SZ);
+        const Rsrc rxl = make_rsrc(S.nxl ? S.xl : vbase, (long long)S.nxl * SZ);
+        const Rsrc rxr = make_rsrc(S.nxr ? S.xr : vbase, (long long)S.nxr * SZ);
+        sigma_lane<T>(rzl, S.nzl, rzr, S.nzr, zin && z <= nz - 2 ? z : -1, nz, true, c.szk2);
+        sigma_lane<T>(rzl, S.nzl, rzr, S.nzr, zin ? z : -1, nz, false, c.szm2);
+        const int xs = min(x0 + (lane < TX ? lane : lane - TX), nx - 1);
+        const int xi = lane < 2 * TX ? (lane < TX ? max(0, min(xs, nx - 2)) : xs) : -1;
+        sigma_lane<T>(rxl, S.nxl, rxr, S.nxr, xi, nx, lane < TX, c.vsig2);
+        march_full<T, TX, R, MODE, PACKED>(c, P, x0);
+    } else {
+        march_int<T, TX, R, MODE, PACKED>(c, P, x0);
+    }
 }
 
 // Receiver sampling: 16 lanes per receiver (4 receivers per wavefront); lane
@@ -651,11 +672,12 @@ __global__ __launch_bounds__(256, PA2D_WAVES_PER_EU) void k_step2d(const pysit_a
     const int gw = xcd_block(b - rec_blocks, gridDim.x - rec_blocks) * 4 + w;
     if (gw >= n_tile_waves) return;
     if (SINGLE) {
-        tile2d<T, TX, R, MODE, SINGLE>(B, P, P.shot0, gw, 1, gw, slot, seg_base, lane, lds_f[w], lds_z[w]);
+        tile2d<T, TX, R, MODE, SINGLE>(B, P, P.shot0, gw, 1, gw, gw / P.shot0.ntz, slot, seg_base, lane, lds_f[w],
+                                       lds_z[w]);
     } else {
         const int4 td = reinterpret_cast<const int4 *>(B.tiles)[gw];
-        tile2d<T, TX, R, MODE, SINGLE>(B, P, B.shots[td.x], gw, td.z & 1, td.y, slot, seg_base, lane, lds_f[w],
-                                       lds_z[w]);
+        tile2d<T, TX, R, MODE, SINGLE>(B, P, B.shots[td.x], gw, td.z & 1, td.y, td.w, slot, seg_base, lane,
+                                       lds_f[w], lds_z[w]);
     }
 }
 

@@ -120,10 +120,14 @@ constexpr int kRsrcWord3 = 0x00020000;  // gfx9-family raw buffer, 32-bit data f
 
 __device__ __forceinline__ Rsrc make_rsrc(const void *base, long long bytes)
 {
-    // the size is wave-uniform; readfirstlane keeps the clamp's result (and so
-    // the whole resource) in SGPRs, where buffer instructions need it
+    // the size is wave-uniform; the clamp to [0, kMaxRsrcBytes] works on the
+    // two 32-bit halves (scalar instructions), and readfirstlane keeps the
+    // result (and so the whole resource) in SGPRs, where buffer instructions
+    // need it
+    const int hi = (int)(bytes >> 32);
+    const unsigned lo = (unsigned)bytes;
     const int n = __builtin_amdgcn_readfirstlane(
-        bytes > kMaxRsrcBytes ? (int)kMaxRsrcBytes : (bytes < 0 ? 0 : (int)bytes));
+        hi < 0 ? 0 : (hi > 0 || lo > (unsigned)kMaxRsrcBytes ? (int)kMaxRsrcBytes : (int)lo));
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, n, kRsrcWord3);
 }
 

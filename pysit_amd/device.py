@@ -267,16 +267,18 @@ class DeviceBatch:
         same = all(s.shape == self.shots[0].shape and s.window == self.shots[0].window
                    for s in self.shots) and len(set(self.kshift)) == 1 and self.tile_order == "interleave"
         rows = []
+        # column 3: the x tile index, t // (tiles per x row)
+        per_x = [ti[0][1] * ti[0][2] for ti in tiles_info]
         if same:
             ntile = len(tiles_info[0][1])
             for t in range(ntile):
                 for si in range(self.n_shots):
-                    rows.append((si, t, int(tiles_info[si][1][t]), 0))
+                    rows.append((si, t, int(tiles_info[si][1][t]), t // per_x[si]))
         else:
             for si in range(self.n_shots):
                 fl = tiles_info[si][1]
                 for t in range(len(fl)):
-                    rows.append((si, t, int(fl[t]), 0))
+                    rows.append((si, t, int(fl[t]), t // per_x[si]))
         tiles = np.asarray(rows, dtype=np.int32).reshape(-1, 4)
         self.n_tiles = tiles.shape[0]
         tile_index = {(int(r[0]), int(r[1])): i for i, r in enumerate(tiles)}
