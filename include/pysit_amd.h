@@ -144,9 +144,10 @@ typedef struct {
     double dt, dx, dy, dz;
     int n_shots;
     const pysit_amd_shot *shots;
-    /* tiles: int4 {shot, tile index within the shot, flags, 0}; flags bit0 =
-     * the tile is within `order/2` nodes of a PML node (full path).  Shots
-     * sorted so that, for every slot, the active tiles form a prefix.        */
+    /* tiles: int4 {shot, tile index t within the shot, flags, x tile index
+     * t / (tiles per x row)}; flags bit0 = the tile is within `order/2` nodes
+     * of a PML node (full path).  Shots sorted so that, for every slot, the
+     * active tiles form a prefix.                                            */
     int n_tiles;
     const int *tiles;
     /* injection, two-level CSR: tile t owns destination nodes
