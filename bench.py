@@ -55,6 +55,9 @@ def parse():
     p.add_argument("--workload", choices=["c2", "c3", "c5"], default="c2",
                    help="c2: the headline 16-shot gradient; c3: 64 shots of uneven aperture and length; "
                         "c5: the multi-band LBFGS FWI loop (FP32, then FP64 for comparison)")
+    p.add_argument("--write-traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                   help="write the traffic summary under profiles/ from the counter files of the two --pmc-run "
+                        "profiles (FETCH_SIZE, WRITE_SIZE) and exit (no GPU needed)")
     p.add_argument("--fwi-iterations", type=int, default=20, help="c5: LBFGS iterations per band")
     p.add_argument("--fwi-bands", default="3,5,8", help="c5: Ricker peak frequencies (Hz), low to high")
     p.add_argument("--pmc-run", action="store_true",
@@ -131,23 +134,91 @@ TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profile
                             "traffic_k_step2d_layered2d.json")
 
 
-def measured_traffic(shots, shape):
+def measured_traffic(shots, shape, tile_x):
     """HBM bytes per launch of the forward and adjoint kernels from the
     committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE per
     dispatch, each scaled by the factor the same counter needs on a copy of
-    known size), when it was measured on this workload; else nulls."""
+    known size), when it was measured on this workload and tile height;
+    else nulls."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return {}
-    if t.get("shots") != shots or list(t.get("grid_padded", [])) != list(shape):
+    if (t.get("shots") != shots or list(t.get("grid_padded", [])) != list(shape)
+            or t.get("tile_x") != tile_x):
         return {}
     out = {"source": os.path.relpath(TRAFFIC_FILE, os.path.dirname(TRAFFIC_FILE) + "/..")}
     for k in ("forward", "adjoint"):
         d = t["kernels"][k]
         out[k] = d["fetch_kib"] * 1024 * t["fetch_factor"] + d["write_kib"] * 1024 * t["write_factor"]
     return out
+
+
+def write_traffic(fetch_csv, write_csv, shots=16, shape=(1041, 341)):
+    """profiles/traffic_k_step2d_layered2d.json from the counter files of
+    the two `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs of
+    `bench.py --pmc-run`: per-dispatch averages of the forward (mode 1, with
+    receivers: the largest grid) and adjoint (mode 4) kernels of the tile
+    height most dispatched, and the factors that make each counter read the
+    calibration copy's 262144 KiB (torch.add over 2^26 floats)."""
+    import csv
+    import collections
+    import re
+
+    def load(path, counter):
+        disp = collections.OrderedDict()
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            d = disp.setdefault(int(r["Dispatch_Id"]), {"name": r["Kernel_Name"], "grid": int(r["Grid_Size"]),
+                                                         "v": 0.0})
+            d["v"] += float(r["Counter_Value"])
+        return list(disp.values())
+
+    pat = re.compile(r"pa::k_step2d<float, (\d+), 4, (\d), false>")
+    out = {}
+    for key, path, counter in (("fetch", fetch_csv, "FETCH_SIZE"), ("write", write_csv, "WRITE_SIZE")):
+        ds = load(path, counter)
+        cal = [d["v"] for d in ds if "_add<float>" in d["name"] and d["grid"] == 2 ** 24]
+        step = collections.defaultdict(list)
+        for d in ds:
+            m = pat.search(d["name"])
+            if m:
+                step[(int(m.group(1)), int(m.group(2)))].append(d)
+        tx = max({k[0] for k in step}, key=lambda t: sum(len(v) for k, v in step.items() if k[0] == t))
+        fwd = step[(tx, 1)]
+        gmax = max(d["grid"] for d in fwd)
+        fwd = [d["v"] for d in fwd if d["grid"] == gmax]
+        adj = [d["v"] for d in step[(tx, 4)]]
+        out[key] = {"cal": sum(cal) / len(cal), "tx": tx, "fwd": (sum(fwd) / len(fwd), len(fwd)),
+                    "adj": (sum(adj) / len(adj), len(adj))}
+    f, w = out["fetch"], out["write"]
+    tx = f["tx"]
+    doc = {
+        "workload": f"2D layered, {shape[0]}x{shape[1]} padded, {shots} shots, order 8, FP32 "
+                    "(bench.py --pmc-run, 40 slots)",
+        "shots": shots, "grid_padded": list(shape), "tile_x": tx,
+        "commands": [
+            "rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmcf -o run --output-format csv -- python3 bench.py "
+            "--pmc-run --steps 1 --warmup 0",
+            "rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmcw -o run --output-format csv -- python3 bench.py "
+            "--pmc-run --steps 1 --warmup 0",
+            "python3 bench.py --write-traffic gpurun_out/pmcf/run_counter_collection.csv "
+            "gpurun_out/pmcw/run_counter_collection.csv"],
+        "calibration": {"kernel": "torch.add(x, 1.0, out=y) over 2^26 float32 (reads and writes 262144 KiB)",
+                        "fetch_kib": f["cal"], "write_kib": w["cal"]},
+        "fetch_factor": round(262144.0 / f["cal"], 4), "write_factor": round(262144.0 / w["cal"], 4),
+        "kernels": {
+            "forward": {"name": f"k_step2d<float,{tx},4,1>", "dispatches": f["fwd"][1],
+                        "fetch_kib": round(f["fwd"][0], 1), "write_kib": round(w["fwd"][0], 1)},
+            "adjoint": {"name": f"k_step2d<float,{tx},4,4>", "dispatches": f["adj"][1],
+                        "fetch_kib": round(f["adj"][0], 1), "write_kib": round(w["adj"][0], 1)}},
+    }
+    with open(TRAFFIC_FILE, "w") as fh:
+        json.dump(doc, fh, indent=1)
+        fh.write("\n")
+    print(json.dumps(doc))
 
 
 def run_uneven(args):
@@ -283,6 +354,9 @@ def run_fwi(args):
 
 def main():
     args = parse()
+    if args.write_traffic:
+        write_traffic(*args.write_traffic)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (shots shard across GPUs "
               f"with one process per GPU, see DESIGN.md)", file=sys.stderr)
@@ -326,12 +400,14 @@ def main():
     v_true = model.C.reshape(nx, nz)
     v0 = gaussian_filter(v_true, sigma=300.0 / mesh.deltas[0], mode="nearest")
     v0 *= v_true.max() / v0.max()
-    tm = pa.TemporalModeling(solver, memory_budget=budget, tile_x=args.tile_x)
+    tm = pa.TemporalModeling(solver, memory_budget=budget, tile_x=args.tile_x, autotune=not args.pmc_run)
     d_obs = torch.from_numpy(np.stack(tm.forward_model_batch(shots, model))).to(dev)
     tm.last_batch = None
 
     m = torch.from_numpy(v0 ** -2.0).to(dev).requires_grad_(True)
-    problem = ShotProblem(solver, shots, memory_budget=budget, tile_x=args.tile_x)
+    # launch configuration: autotuned, except in the counter run, which keeps
+    # the default one (counter collection distorts the tuner's timings)
+    problem = ShotProblem(solver, shots, memory_budget=budget, tile_x=args.tile_x, autotune=not args.pmc_run)
 
     last = {}
 
@@ -370,7 +446,7 @@ def main():
 
     fwd_ms, n_fwd = avg_ms(1)
     adj_ms, n_adj = avg_ms(2)
-    traffic = measured_traffic(len(shots), shape)
+    traffic = measured_traffic(len(shots), shape, problem.batch.tile_x)
     itemsize = 4 if solver.dtype == np.single else 8
     npml = pml_nodes(shape, pads)
     b_fwd, b_adj = algorithmic_bytes([(npts, npml)] * len(shots), itemsize)
@@ -395,13 +471,15 @@ def main():
                                f"{len(shots)} shots x 1001 receivers, nt={nt}, FP32, batched fused launches, "
                                f"dWaveOp segment {segment} of {nt} slots",
                    "grid_padded": list(shape), "shots": len(shots), "nt": nt, "order": 8,
-                   "tile_x": problem.batch.tile_x},
+                   "tile_x": problem.batch.tile_x, "tile_order": problem.batch.tile_order,
+                   "graph_steps": problem.batch.graph_steps},
         "roofline": {"bound": "hbm", "achieved": ach_adj, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach_adj / HBM_PEAK_GBS, "traffic": traffic.get("adjoint"),
-                     "kernel": ("k_step2d<float,4,4> (adjoint + imaging, FP32 window sums into FP64)"
-                                if problem.batch.ic_window else "k_step2d<float,4,2> (adjoint + FP64 imaging)"),
+                     "kernel": (f"k_step2d<float,{problem.batch.tile_x},4,4> (adjoint + imaging, FP32 window "
+                                "sums into FP64)" if problem.batch.ic_window else
+                                f"k_step2d<float,{problem.batch.tile_x},4,2> (adjoint + FP64 imaging)"),
                      "bytes_per_launch": b_adj, "avg_launch_us": adj_ms * 1e3, "launches": n_adj},
-        "forward_kernel": {"kernel": "k_step2d<float,4,1> (forward + dWaveOp store)",
+        "forward_kernel": {"kernel": f"k_step2d<float,{problem.batch.tile_x},4,1> (forward + dWaveOp store)",
                            "achieved_gbs": ach_fwd, "frac": ach_fwd / HBM_PEAK_GBS,
                            "avg_launch_us": fwd_ms * 1e3, "bytes_per_launch": b_fwd, "launches": n_fwd,
                            "traffic": traffic.get("forward")},

@@ -14,7 +14,8 @@ grids and the number of shots:
 :func:`tuned_batch` builds the batch in each candidate configuration, times a
 short forward-with-dWaveOp plus adjoint-imaging pass on the device
 (:meth:`DeviceBatch.probe_step_time`) and returns the batch in the fastest
-one.  Choices are cached per signature -- dimension, dtype, order, imaging
+one; a candidate must beat the default (or the best so far) by ``MARGIN`` to
+be chosen, so near ties resolve the same way in every run.  Choices are cached per signature -- dimension, dtype, order, imaging
 period, FP32 window and every shot's padded grid, PML widths and receiver
 count, not the step count, so an inversion whose dt changes keeps its choice
 -- in memory and, when the environment variable ``PYSIT_AMD_TUNE_CACHE``
@@ -32,6 +33,9 @@ __all__ = ["tuned_batch", "signature", "candidates", "clear_cache"]
 
 _CACHE = {}
 GRAPH_STEPS = 32
+# a candidate replaces the one before it (the default first) only when it is
+# this much faster: near ties keep the default, so repeated runs choose alike
+MARGIN = 0.03
 
 
 def signature(shots, dtype, order, imaging_period=1, ic_window=None):
@@ -99,7 +103,11 @@ def _store(sig, cfg):
         pass
 
 
-def tuned_batch(shots, model, deltas, dt, order, dtype=np.float32, slots=16, report=None, **kw):
+def _faster(t, best_t):
+    return t < best_t * (1.0 - MARGIN)
+
+
+def tuned_batch(shots, model, deltas, dt, order, dtype=np.float32, slots=32, report=None, **kw):
     """A :class:`DeviceBatch` of ``shots`` in the fastest launch
     configuration (keyword arguments other than the tuned ones pass through).
     ``report``, a list, receives (configuration, seconds per slot) of every
@@ -115,12 +123,15 @@ def tuned_batch(shots, model, deltas, dt, order, dtype=np.float32, slots=16, rep
             b = DeviceBatch(shots, model, deltas, dt, order, dtype=dtype, graph_steps=0, **cand, **kw)
             timed.append((b.probe_step_time(slots), dict(cand, graph_steps=0)))
             del b
-        best_t, best = min(timed, key=lambda x: x[0])
+        best_t, best = timed[0]
+        for t, c in timed[1:]:
+            if _faster(t, best_t):
+                best_t, best = t, c
         b = DeviceBatch(shots, model, deltas, dt, order, dtype=dtype,
                         **dict(best, graph_steps=GRAPH_STEPS), **kw)
         tg = b.probe_step_time(slots)
         timed.append((tg, dict(best, graph_steps=GRAPH_STEPS)))
-        if tg < best_t:
+        if _faster(tg, best_t):
             cfg = dict(best, graph_steps=GRAPH_STEPS)
             if report is not None:
                 report.extend((c, t) for t, c in timed)
