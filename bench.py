@@ -52,8 +52,9 @@ def parse():
     p.add_argument("--tile-x", type=int, default=None)
     p.add_argument("--memory-budget-gb", type=float, default=None)
     p.add_argument("--profile-run", action="store_true", help="minimal run for rocprofv3 (no plain, no CPU)")
-    p.add_argument("--workload", choices=["c2", "c3", "c5"], default="c2",
+    p.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2",
                    help="c2: the headline 16-shot gradient; c3: 64 shots of uneven aperture and length; "
+                        "c4: the 3D 256^3 8-shot checkpointed gradient (use --steps 1 --warmup 0); "
                         "c5: the multi-band LBFGS FWI loop (FP32, then FP64 for comparison)")
     p.add_argument("--write-traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                    help="write the traffic summary under profiles/ from the counter files of the two --pmc-run "
@@ -66,19 +67,20 @@ def parse():
     return p.parse_args()
 
 
-def algorithmic_bytes(shapes_and_pml, itemsize):
+def algorithmic_bytes(shapes_and_pml, itemsize, dim=2):
     """Algorithmic HBM bytes of one launch over all shots, the path's byte
     model (s = itemsize, per padded node per time step):
 
     forward: read u_k, u_{k-1} and the coefficient, write u_{k+1} (4s);
     adjoint: the same (4s) plus the imaging condition, one stored forward
     quantity read (s) and the FP64 accumulator read and written (16).  PML
-    nodes add Phix and Phiz read and written (4s) in either direction.
-    For FP32 this is 26 + 16 f_pml bytes per gradient grid-point update."""
+    nodes add Phix and Phiz read and written (4s) in either direction, in 3D
+    Phix, Phiy, Phiz and psi (8s).  For FP32 this is 26 + 16 f_pml bytes per
+    gradient grid-point update in 2D, 26 + 32 f_pml in 3D."""
     fwd = adj = 0.0
     for npts, npml in shapes_and_pml:
         base = 4 * itemsize * npts
-        pml = 4 * itemsize * npml
+        pml = (4 if dim == 2 else 8) * itemsize * npml
         fwd += base + pml
         adj += base + pml + (itemsize + 16) * npts
     return fwd, adj
@@ -286,6 +288,112 @@ def run_uneven(args):
         "host_setup_s": t_setup}), flush=True)
 
 
+def run_3d(args):
+    """The 3D case: 256^3 nodes at dx = 12.5 m (288^3 padded with 16 PML
+    cells per side), velocities 1500-4000 m/s in seeded layers with a -10 %
+    Gaussian anomaly (radius 400 m), 8 Ricker sources (6 Hz) in a line at 25 m
+    depth, a 64 x 64 receiver patch at 25 m, nt = 2000 at the CFL step, FP32,
+    order 8, in km and km/s (the 3D PML of the reference's step is stable in
+    these units and not in m and m/s, see workloads.layered_3d).  One step is the gradient of 0.5 dt ||d||^2 (residual = the
+    traces) over all 8 shots in one batch; the dWaveOp history does not fit,
+    so the forward keeps checkpoints and the adjoint recomputes each segment
+    (segment length from 80 % of free HBM).  value = 2 sum N nt / s, the
+    recompute not counted.  FP32 against FP64 is measured on the first shot
+    alone (eight FP64 shots do not fit a one-level checkpoint schedule in
+    288 GB).  Prints one JSON line."""
+    import torch
+    from pysit_amd.device import DeviceBatch
+    from pysit_amd.modeling import TemporalModeling
+    from pysit_amd.workloads import layered_3d
+    t0 = time.perf_counter()
+    nt = args.nt if args.nt != 3000 else 2000
+    mesh, shots, solver, model = layered_3d(n=256, n_shots=8, nt=nt)
+    tm = TemporalModeling(solver, autotune=False)
+    geos = [tm.geometry(sh) for sh in shots]
+    t_setup = time.perf_counter() - t0
+    shape = tuple(mesh.shape(include_bc=True, as_grid=True))
+    npts = int(np.prod(shape))
+    npml = pml_nodes(shape, mesh.pad_widths())
+    model_u = solver.unpadded_model()
+    budget = int(args.memory_budget_gb * 2 ** 30) if args.memory_budget_gb else None
+
+    def gradient(b, res, timed):
+        reps = (args.warmup + args.steps) if timed else 1
+        times = []
+        for i in range(reps):
+            keep = timed and i == reps - 1
+            b.timer = {} if keep else None
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            b.forward(keep_dw=True)
+            b.migrate(res)
+            e1.record()
+            torch.cuda.synchronize()
+            if i >= reps - (args.steps if timed else 1):
+                times.append(e0.elapsed_time(e1) * 1e-3)
+            print(f"[c4] gradient {i + 1}/{reps}: {times[-1] if times else e0.elapsed_time(e1) * 1e-3:.2f} s, "
+                  f"segment {b.K}", file=sys.stderr, flush=True)
+        return sum(times) / len(times)
+
+    def avg_us(timer, mode):
+        tot = sum(a.elapsed_time(e) for a, e, _ in timer.get(mode, []))
+        n = sum(c for _, _, c in timer.get(mode, []))
+        return tot * 1e3 / max(n, 1), n
+
+    b = DeviceBatch(geos, model_u, mesh.deltas, solver.dt, 8, dtype=np.float32, memory_budget=budget)
+    res = [t.clone() for t in b.forward(keep_dw=False)]
+    t32 = gradient(b, res, True)
+    timer = b.timer
+    adj_us, n_adj = avg_us(timer, 2)
+    fwd_us, n_fwd = avg_us(timer, 1)
+    b_fwd, b_adj = algorithmic_bytes([(npts, npml)] * len(shots), 4, dim=3)
+    K = b.K
+    img32 = b.image(0).cpu().numpy().reshape(-1).copy()
+    tr32 = res[0].cpu().numpy().copy()
+    del b
+    torch.cuda.empty_cache()
+    acc = None
+    if not args.no_accuracy:
+        b32 = DeviceBatch(geos[:1], model_u, mesh.deltas, solver.dt, 8, dtype=np.float32, memory_budget=budget)
+        r32 = [t.clone() for t in b32.forward(keep_dw=False)]
+        gradient(b32, r32, False)
+        i32 = b32.image(0).cpu().numpy().reshape(-1).copy()
+        del b32
+        torch.cuda.empty_cache()
+        b64 = DeviceBatch(geos[:1], model_u, mesh.deltas, solver.dt, 8, dtype=np.float64, memory_budget=budget)
+        r64 = [r32[0].to(torch.float64)]
+        tr64 = b64.forward(keep_dw=False)[0].cpu().numpy().copy()
+        gradient(b64, r64, False)
+        i64 = b64.image(0).cpu().numpy().reshape(-1).copy()
+        del b64
+        torch.cuda.empty_cache()
+
+        def rel(a, c):
+            return float(np.linalg.norm(a - c) / np.linalg.norm(c))
+        acc = {"traces_rel_l2": rel(r32[0].cpu().numpy(), tr64), "gradient_rel_l2": rel(i32, i64),
+               "note": "first shot, FP32 against the FP64 device path on the same residual"}
+    work = 2.0 * npts * nt * len(shots)
+    value = work / t32 / 1e9
+    ach = b_adj / (adj_us * 1e-6) / 1e9
+    print(json.dumps({
+        "metric": "C4 3D fwd+adjoint gradient Gpt-updates/s (8 shots, checkpointed)",
+        "value": value, "unit": "Gpt-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t32 * 1e3, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "3D layered + anomaly, 256^3 (288^3 padded), dx=12.5 m, order 8, PML 16, 8 shots, "
+                               "64x64 receivers, FP32, checkpointed; units km, km/s, s",
+                   "shots": len(shots), "nt": nt, "grid_padded": list(shape), "dwaveop_segment": K,
+                   "checkpoints": -(-nt // K)},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "k_step3d<float,4,2> (adjoint + imaging)",
+                     "bytes_per_launch": b_adj, "avg_launch_us": adj_us, "launches": n_adj},
+        "forward_kernel": {"kernel": "k_step3d<float,4,1> (recompute + dWaveOp store)", "avg_launch_us": fwd_us,
+                           "achieved_gbs": b_fwd / (fwd_us * 1e-6) / 1e9, "bytes_per_launch": b_fwd,
+                           "launches": n_fwd},
+        "gradient_roofline_frac": value * 1e9 * (b_fwd + b_adj) / (2 * npts * len(shots)) / 1e9 / HBM_PEAK_GBS,
+        "accuracy_vs_fp64": acc, "host_setup_s": t_setup}), flush=True)
+
+
 def run_fwi(args):
     """The 2D FWI loop: 601 x 201 nodes at dx = 15 m (641 x 241 padded), the
     layered true model, its 300 m Gaussian smoothing as the start model, 32
@@ -367,6 +475,9 @@ def main():
     from pysit_amd.workloads import layered_2d
     if args.workload == "c3":
         run_uneven(args)
+        return
+    if args.workload == "c4":
+        run_3d(args)
         return
     if args.workload == "c5":
         run_fwi(args)

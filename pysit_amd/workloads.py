@@ -157,12 +157,20 @@ def layered_velocity_3d(n, dx, vmin=1500.0, vmax=4000.0, n_layers=8, seed=0, ano
 
 
 def layered_3d(n=256, dx=12.5, n_shots=8, nt=2000, order=8, precision="single", pml_cells=16, f0=6.0,
-               rec_patch=64, depth=None, seed=0):
+               rec_patch=64, depth=None, seed=0, pml_reflection=1e-3, unit=1e-3):
     """The 3D case: n^3 nodes, ``n_shots`` sources in a line at the surface,
-    a rec_patch x rec_patch receiver grid at the surface."""
-    v = layered_velocity_3d((n, n, n), dx, seed=seed)
+    a rec_patch x rec_patch receiver grid at the surface.  Lengths and
+    speeds are given in metres and m/s and scaled by ``unit``; times stay in
+    seconds.  The default, km and km/s, matters: the 3D step multiplies the
+    sigma_x sigma_y sigma_z psi term by C^2
+    (constant_density_acoustic_time_scalar_3D.h:218), so its PML depends on
+    the unit of speed.  With C near 4 km/s it absorbs; with C near 4000 m/s
+    the PML corners grow without bound within a few hundred steps (the
+    reference's own arithmetic, reproduced by the oracle and the kernels)."""
+    v = layered_velocity_3d((n, n, n), dx, seed=seed) * unit
+    dx = dx * unit
     width = pml_cells * dx
-    A = pml_amplitude(v.max(), width)
+    A = pml_amplitude(v.max(), width, pml_reflection)
     p = PML(width, A)
     L = (n - 1) * dx
     d = RectangularDomain((0.0, L, p, p), (0.0, L, p, p), (0.0, L, p, p))

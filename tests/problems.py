@@ -39,7 +39,7 @@ class Synthetic:
     """One shot on a padded grid with PML of widths `pml` per axis side."""
 
     def __init__(self, shape, pml, deltas, nt, order, seed=0, amp_pml=60.0, n_src=1, n_rec=7,
-                 window=None, model=None, dt=None):
+                 window=None, model=None, dt=None, src_center=None, rec_z=None):
         rng = np.random.default_rng(seed)
         self.shape = tuple(shape)
         dim = len(shape)
@@ -72,6 +72,8 @@ class Synthetic:
         pos_all, chan_all, w_all = [], [], []
         for c in range(n_src):
             center = [int(rng.integers(pml[i][0] + 2, shape[i] - pml[i][1] - 2)) for i in range(dim)]
+            if src_center is not None:
+                center = list(src_center)
             pos, w = gaussian_patch(center, shape, 2, rng, dim)
             pos_all.append(pos)
             chan_all.append(np.full(pos.shape[0], c))
@@ -89,7 +91,7 @@ class Synthetic:
         self.amp = amp
         # receivers: a line along x (2D) or a patch (3D) near the top
         rec_pos, rec_chan, rec_w = [], [], []
-        zr = pml[-1][0] + 2
+        zr = pml[-1][0] + 2 if rec_z is None else int(rec_z)
         for r in range(n_rec):
             if dim == 2:
                 xr = int(np.linspace(pml[0][0] + 1, shape[0] - pml[0][1] - 2, n_rec)[r])

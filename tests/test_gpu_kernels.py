@@ -224,6 +224,53 @@ def test_batch_3d(gpu, dtype, tol):
     assert rel_l2(img, img_ref) < tol * 10
 
 
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-10), (np.float32, 1e-3)])
+def test_batch_3d_with_plain_tiles(gpu, dtype, tol):
+    # 32 x 36 x 140 nodes with 4-node PML: the x-y tiles in the middle and the
+    # z tile at 64..127 are plain tiles (no PML node within r of them); the
+    # source and the receivers sit in them, and the whole last wavefield is
+    # compared too
+    p = Synthetic((32, 36, 140), ((4, 4), (4, 4), (4, 4)), (0.01, 0.011, 0.012), 60, 8, seed=7, n_rec=9,
+                  src_center=(16, 18, 92), rec_z=96)
+    b = _batch([p], dtype)
+    flags = b.tiles.view(-1, 4)[:, 2].cpu().numpy()
+    assert (flags == 0).sum() >= 4
+    tr = b.forward(keep_dw=True, store_field=True)[0].cpu().numpy()
+    field = b.shot_field(b.field_store, 0, p.nt - 1).cpu().numpy().reshape(-1)
+    op = p.oracle_problem()
+    fw = op.forward(wavefield=True)
+    ref = fw["simdata"]
+    assert np.linalg.norm(ref) > 1e-3 * np.abs(p.amp).max()
+    assert rel_l2(tr, ref) < tol
+    assert rel_l2(field, fw["wavefield"][p.nt - 1]) < tol
+    rng = np.random.default_rng(3)
+    res = [rng.standard_normal(ref.shape)]
+    b.migrate(res)
+    img = b.image(0).cpu().numpy().reshape(-1) * (-p.dt)
+    img_ref = op.migrate(res[0], fold=False)
+    assert rel_l2(img, img_ref) < tol * 10
+
+
+def test_layered_3d_workload_is_stable_and_fp32_follows_fp64(gpu):
+    # the 3D workload (km, km/s) on 48^3 nodes + 16 PML cells, 600 steps: the
+    # traces stay bounded and FP32 follows FP64
+    from pysit_amd.device import DeviceBatch
+    from pysit_amd.modeling import TemporalModeling
+    from pysit_amd.workloads import layered_3d
+    mesh, shots, solver, model = layered_3d(n=48, n_shots=1, nt=600, rec_patch=16)
+    tm = TemporalModeling(solver, autotune=False)
+    g = tm.geometry(shots[0])
+    trs = []
+    for dtype in (np.float64, np.float32):
+        b = DeviceBatch([g], solver.unpadded_model(), mesh.deltas, solver.dt, 8, dtype=dtype)
+        trs.append(b.forward(keep_dw=False)[0].cpu().numpy())
+    t64, t32 = trs
+    assert np.all(np.isfinite(t64)) and np.all(np.isfinite(t32))
+    late, peak = np.abs(t64[-100:]).max(), np.abs(t64).max()
+    assert late < peak
+    assert rel_l2(t32, t64) < 1e-4
+
+
 def test_fold_matches_oracle(gpu):
     import torch
     from pysit_amd import _native as N
