@@ -114,10 +114,12 @@ int pysit_amd_cda_step3d_f64(int order, const double *km1_u, const double *k_phi
 /* Per-shot descriptor (device array, one per shot). */
 typedef struct {
     int nx, ny, nz;        /* padded grid of this shot (ny = 1 in 2D)            */
+    int zs;                /* row stride along z in the batch's arrays, >= nz: node
+                            * (i,j,k) sits at off + (i*ny + j)*zs + k            */
     int nt;                /* number of time steps                               */
     int kshift;            /* local step k = slot - kshift                       */
     int ntx, nty, ntz;     /* tiles of this shot along x, y, z                   */
-    long long off;         /* element offset of this shot in the field arrays    */
+    long long off;         /* element offset of this shot in the batch's arrays  */
     /* model (wave speed, dtype of the batch): value at padded (i,j,k) is
      * model[((clamp(i-mx_off,mx_lo,mx_hi))*mny + clamp(j-my_off,..))*mnz
      *        + clamp(k-mz_off,mz_lo,mz_hi)]  -- edge padding by clamping,

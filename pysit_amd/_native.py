@@ -20,7 +20,7 @@ _d = ctypes.c_double
 class Shot(ctypes.Structure):
     """``pysit_amd_shot``."""
     _fields_ = [
-        ("nx", _i), ("ny", _i), ("nz", _i), ("nt", _i), ("kshift", _i),
+        ("nx", _i), ("ny", _i), ("nz", _i), ("zs", _i), ("nt", _i), ("kshift", _i),
         ("ntx", _i), ("nty", _i), ("ntz", _i),
         ("off", _ll),
         ("model", _vp), ("mny", _i), ("mnz", _i),

@@ -492,7 +492,7 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
     // z tiles: 0, 64, ... and a last one ending on node nz-1; owned ranges disjoint
     const int ntz = S.ntz;
     const int tz = tloc - tx * ntz;
-    const int zlast = nz > 64 ? nz - 64 : 0;
+    const int zlast = z_last_tile(nz, z_align<T>());
     const int z0 = (tz == ntz - 1) ? zlast : tz * 64;
     const int own_hi = (tz == ntz - 1) ? nz : min(z0 + 64, zlast);
     const int x0 = tx * TX;
@@ -518,7 +518,7 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
     // disabled arrays get an empty resource on a valid base: every access is
     // out of range, loads return 0 and stores are dropped
     const void *vbase = B.u[0];
-    c.rowb = nz * SZ;
+    c.rowb = S.zs * SZ;
     c.shb = (int)(S.off * SZ);
     c.shb8 = (int)(S.off * 8);
     if constexpr (PACKED) {
@@ -530,7 +530,7 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
         c.s.o[F_PZ] = P.fo[2][cur] + c.shb;
         c.s.o[F_PZO] = P.fo[2][cur ^ 1] + c.shb;
     } else {
-        const long long nb = (long long)nx * nz * SZ;
+        const long long nb = (long long)nx * S.zs * SZ;
         c.s.r[F_U] = make_rsrc(B.u[cur], nb);
         c.s.r[F_UO] = make_rsrc(B.u[cur ^ 1], nb);
         c.s.r[F_PX] = make_rsrc(B.phix[cur], nb);
@@ -882,6 +882,7 @@ int step2d_entry(int order, const T *km1_u, const T *k_phix, const T *k_phiz, co
     S.nx = nx;
     S.ny = 1;
     S.nz = nz;
+    S.zs = nz;
     S.nt = 1;
     S.kshift = 0;
     S.ntz = (nz + 63) / 64;

@@ -376,7 +376,8 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
     // tiles: z fastest, then y, then x; z tiles as in 2D (the last ends on nz-1)
     const int ntz = S.ntz;
     const int tz = tloc % ntz, tyi = (tloc / ntz) % S.nty, tx = tloc / (ntz * S.nty);
-    const int zlast = nz > 64 ? nz - 64 : 0;
+    const int zlast = z_last_tile(nz, z_align<T>());
+    const int zs = S.zs;
     const int z0 = (tz == ntz - 1) ? zlast : tz * 64;
     const int own_hi = (tz == ntz - 1) ? nz : min(z0 + 64, zlast);
     const int y0 = tyi * TY3, x0 = tx * TX;
@@ -384,13 +385,13 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
     c.y = y;
     const bool zin = z < nz, yin = y < ny;
     const bool own = zin && z < own_hi && yin;
-    c.plane = ny * nz * SZ;
-    c.co = (zin && yin) ? (y * nz + z) * SZ : kOOB;
-    c.ov = own ? (y * nz + z) * SZ : kOOB;
-    c.ov8 = own ? (y * nz + z) * 8 : kOOB;
+    c.plane = ny * zs * SZ;
+    c.co = (zin && yin) ? (y * zs + z) * SZ : kOOB;
+    c.ov = own ? (y * zs + z) * SZ : kOOB;
+    c.ov8 = own ? (y * zs + z) * 8 : kOOB;
     const int hz = lane < R ? z0 + 64 + lane : (lane >= 64 - R ? z0 + lane - 64 : -1);
     const bool hin = hz >= 0 && hz < nz;
-    c.cho = (hin && yin) ? (y * nz + hz) * SZ : kOOB;
+    c.cho = (hin && yin) ? (y * zs + hz) * SZ : kOOB;
     // LDS column of the z-halo value, own row in the u/psi planes and the Phiz plane
     const int hcol = lane < R ? R + 64 + lane : (lane >= 64 - R ? lane - (64 - R) : -1);
     c.hl = hcol >= 0 ? (R + w) * LZ + hcol : -1;
@@ -398,12 +399,12 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
     // y-halo rows: waves 0..r-1 above the tile, waves r..2r-1 below it
     c.hrow = w < R ? w : (w < 2 * R ? TY3 + w : -1);
     const int hy = y0 - R + c.hrow;
-    c.hro = (c.hrow >= 0 && hy >= 0 && hy < ny && zin) ? (hy * nz + z) * SZ : kOOB;
+    c.hro = (c.hrow >= 0 && hy >= 0 && hy < ny && zin) ? (hy * zs + z) * SZ : kOOB;
     c.yzedge = y == 0 || y >= ny - 1 || z == 0 || z >= nz - 1;
     const int taken = (MODE == 2) ? (S.nt - 1 - k) : k;
     const int cur = taken & 1;
     const long long off = S.off;
-    const long long nb = (long long)nx * ny * nz * SZ;
+    const long long nb = (long long)nx * ny * zs * SZ;
     const T *vb = (const T *)B.u[0];
     c.u = make_rsrc((const T *)B.u[cur] + off, nb);
     c.uo = make_rsrc((T *)B.u[cur ^ 1] + off, nb);
@@ -668,6 +669,7 @@ int step3d_entry(int order, const T *km1_u, const T *k_phix, const T *k_phiy, co
     S.nx = nx;
     S.ny = ny;
     S.nz = nz;
+    S.zs = nz;
     S.nt = 1;
     S.kshift = 0;
     S.ntz = (nz + 63) / 64;

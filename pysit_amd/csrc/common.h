@@ -105,6 +105,19 @@ __device__ __forceinline__ void wave_sync()
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// z tiles of an axis of nz nodes (2D and 3D): 64 nodes each, starting at 0,
+// 64, 128, ... except the last, which starts at the first multiple of
+// `align` nodes at or after nz - 64 (so it still reaches node nz - 1, and
+// with rows stored at an aligned stride its accesses start on a cache line);
+// each tile owns the nodes up to the next tile's start.  The host builds the
+// same tiles (pysit_amd/device.py z_tiles).
+__host__ __device__ constexpr int z_last_tile(int nz, int align)
+{
+    return nz > 64 ? ((nz - 64 + align - 1) / align) * align : 0;
+}
+// row alignment in nodes: one 128-byte cache line
+template <typename T> constexpr int z_align() { return 128 / (int)sizeof(T); }
+
 // ---- raw buffer access -----------------------------------------------------
 // A buffer resource covers `bytes` from `base`.  An access names a per-lane
 // byte offset (VGPR) and a wave-uniform byte offset (SGPR, e.g. the row and

@@ -30,12 +30,24 @@ from . import _native as N
 TILE_Z = 64      # z nodes per wavefront (the contiguous axis)
 
 
-def z_tiles_2d(nz):
+def z_last_tile(nz, align):
+    """Start of the last z tile: the first multiple of ``align`` nodes at or
+    after nz - 64 (0 when nz <= 64), common.h z_last_tile."""
+    return -(-(nz - TILE_Z) // align) * align if nz > TILE_Z else 0
+
+
+def z_align(itemsize):
+    """Row alignment in nodes: one 128-byte cache line."""
+    return 128 // itemsize
+
+
+def z_tiles_2d(nz, align=1):
     """z tiles (2D and 3D): (start, owned end) per tile.  Tiles start at 0, 64,
-    128, ... except the last, which ends on node nz-1; each node is owned by
-    exactly one tile (the kernels' tiling, cda2d.hip tile2d, cda3d.hip tile3d)."""
+    128, ... except the last, which starts at z_last_tile(nz, align) and
+    reaches node nz-1; each node is owned by exactly one tile (the kernels'
+    tiling, cda2d.hip tile2d, cda3d.hip tile3d)."""
     ntz = -(-nz // TILE_Z)
-    zlast = max(0, nz - TILE_Z)
+    zlast = z_last_tile(nz, align)
     out = []
     for tz in range(ntz):
         if tz == ntz - 1:
@@ -144,6 +156,7 @@ class DeviceBatch:
         else:
             self.tile_x = N.tile_rows(self.dim, code, 0)
         self.tile_y = N.tile_rows(self.dim, 1 if self.dtype == np.float64 else 0, 1) if self.dim == 3 else 1
+        self.zalign = z_align(self.dtype.itemsize)
         self.memory_budget = memory_budget
         # FP32 2D adjoints sum the imaging condition in FP32 over windows of
         # `ic_window` slots and in FP64 across windows (0: FP64 every step)
@@ -198,7 +211,7 @@ class DeviceBatch:
             # a tile takes the plain path only if no node within R of the
             # nodes it owns is a PML node, an edge node or outside the grid
             nx, nz = s.shape
-            zt = z_tiles_2d(nz)
+            zt = z_tiles_2d(nz, self.zalign)
             ntz = len(zt)
             ntx = -(-nx // self.tile_x)
             flags = np.zeros(ntx * ntz, dtype=np.int32)
@@ -212,7 +225,7 @@ class DeviceBatch:
                     flags[tx * ntz + tz] = 1 if (xp or zp) else 0
             return (ntx, 1, ntz), flags
         nx, ny, nz = s.shape
-        zt = z_tiles_2d(nz)
+        zt = z_tiles_2d(nz, self.zalign)
         ntz = len(zt)
         nty = -(-ny // self.tile_y)
         ntx = -(-nx // self.tile_x)
@@ -234,14 +247,14 @@ class DeviceBatch:
         ntx, nty, ntz = s_tiles
         if self.dim == 2:
             nz = s.shape[1]
-            zlast = max(0, nz - TILE_Z)
+            zlast = z_last_tile(nz, self.zalign)
             z = pos[:, 1]
             tz = np.where(z >= zlast, ntz - 1, z // TILE_Z)
             z0 = np.where(tz == ntz - 1, zlast, tz * TILE_Z)
             return (pos[:, 0] // self.tile_x) * ntz + tz, (pos[:, 0] % self.tile_x) * TILE_Z + (z - z0)
         ty = self.tile_y
         nz = s.shape[2]
-        zlast = max(0, nz - TILE_Z)
+        zlast = z_last_tile(nz, self.zalign)
         z = pos[:, 2]
         tz = np.where(z >= zlast, ntz - 1, z // TILE_Z)
         z0 = np.where(tz == ntz - 1, zlast, tz * TILE_Z)
@@ -252,12 +265,19 @@ class DeviceBatch:
     def _build_tables(self):
         dev = self.dev
         T = self.dtype
-        self.offs = []
+        # storage: every shot's rows along z at a stride of whole cache lines
+        # (zs >= nz; the nodes past nz are never read or written)
+        self.offs, self.zs, self.snpts = [], [], []
         off = 0
         tiles_info = []
         for s in self.shots:
+            nz = s.shape[-1]
+            zs = -(-nz // self.zalign) * self.zalign
+            sn = int(np.prod(s.shape[:-1])) * zs
             self.offs.append(off)
-            off += s.npts
+            self.zs.append(zs)
+            self.snpts.append(sn)
+            off += sn
             tiles_info.append(self._tiles_of(s))
         self.total_n = off
         self.tiles_info = tiles_info
@@ -308,6 +328,8 @@ class DeviceBatch:
         e = 0
         for si, s in enumerate(self.shots):
             idx, chan, w = s.rec
+            nz = s.shape[-1]
+            idx = (np.asarray(idx, dtype=np.int64) // nz) * self.zs[si] + np.asarray(idx, dtype=np.int64) % nz
             order = np.argsort(chan, kind="stable")
             idx, chan, w = idx[order], chan[order], w[order]
             counts = np.bincount(chan, minlength=s.nrec)
@@ -384,6 +406,7 @@ class DeviceBatch:
             d.nx = s.shape[0]
             d.ny = s.shape[1] if self.dim == 3 else 1
             d.nz = s.shape[-1]
+            d.zs = self.zs[si]
             d.nt = s.nt
             d.kshift = self.kshift[si]
             ntx, nty, ntz = self.tiles_info[si][0]
@@ -579,18 +602,25 @@ class DeviceBatch:
                 if k >= s.nt:
                     break
                 slot = k + self.kshift[pos]
-                src = torch.as_tensor(np.ascontiguousarray(np.asarray(a).reshape(-1), dtype=self.dtype))
-                self.dw[slot, self.offs[pos]:self.offs[pos] + s.npts].copy_(src.to(self.dev))
+                src = torch.as_tensor(np.ascontiguousarray(np.asarray(a).reshape(s.shape), dtype=self.dtype))
+                self._shot_view(self.dw[slot], pos).copy_(src.to(self.dev))
         self.K = self.max_nt
         self.ckpts = []
         self.have_dw = True
 
+    def _shot_view(self, flat, pos):
+        """The padded grid of shot ``pos`` inside a flat batch array (a strided
+        view: rows along z are stored at the stride zs >= nz)."""
+        s = self.shots[pos]
+        zs = self.zs[pos]
+        grid = flat[self.offs[pos]:self.offs[pos] + self.snpts[pos]].view(tuple(s.shape[:-1]) + (zs,))
+        return grid[..., :s.shape[-1]]
+
     def shot_field(self, store, si_caller, k):
         """Padded field of time level k of one shot from a field store."""
         pos = self.order_idx.index(si_caller)
-        s = self.shots[pos]
         slot = k + self.kshift[pos]
-        return store[slot, self.offs[pos]:self.offs[pos] + s.npts].view(s.shape)
+        return self._shot_view(store[slot], pos)
 
     def shot_dw(self, si_caller, k):
         """Stored dWaveOp of step k of one shot (store-all histories only)."""
@@ -728,9 +758,7 @@ class DeviceBatch:
 
     def image(self, si_caller, scale=1.0):
         """Padded FP64 image of one shot (caller order), a view of the accumulator."""
-        pos = self.order_idx.index(si_caller)
-        s = self.shots[pos]
-        return self.ic[self.offs[pos]:self.offs[pos] + s.npts].view(s.shape)
+        return self._shot_view(self.ic, self.order_idx.index(si_caller))
 
     def fold(self, si_caller, pads, out, out_origin=None, scale=1.0, accumulate=True):
         """Fold one shot's image onto ``out`` (device FP64 tensor holding the
@@ -738,7 +766,7 @@ class DeviceBatch:
         node in ``out`` (default 0)."""
         pos = self.order_idx.index(si_caller)
         s = self.shots[pos]
-        img = self.ic[self.offs[pos]:]
+        img = self._shot_view(self.ic, pos).contiguous()  # the fold reads a dense padded grid
         origin = out_origin or (0,) * self.dim
         if self.dim == 2:
             (lx, rx), (lz, rz) = pads
