@@ -354,9 +354,40 @@ class DeviceBatch:
             self.pml_dev.append(arrs)
         self.amp_dev = [torch.from_numpy(s.amp).to(dev) for s in self.shots]
         self.traces = [torch.zeros((s.nt, s.nrec), dtype=torch.float64, device=dev) for s in self.shots]
+        self._model_layout()
         self.shots_fwd = self._shot_table(self.amp_dev, [s.amp.shape[1] for s in self.shots])
         self.residual_dev = None
         self.shots_adj = None
+
+    def _model_layout(self):
+        """The kernels' copy of the model.  When every shot takes the same z
+        window of the model, the copy lays each model row along the padded z
+        axis at the batch's row stride (edge values by clamping, as the
+        window does), so model rows start on the same cache-line offsets as
+        the field rows; otherwise it is the model itself."""
+        zw = {tuple(int(v) for v in s.window[-1]) for s in self.shots}
+        nzs = {s.shape[-1] for s in self.shots}
+        self._mz = None
+        if len(zw) == 1 and len(nzs) == 1:
+            off, lo, hi = zw.pop()
+            nz = nzs.pop()
+            zs = -(-nz // self.zalign) * self.zalign
+            idx = torch.clamp(torch.arange(zs, device=self.dev) - off, lo, hi)
+            self._mz = (idx, zs)
+        self.model_k = self._kernel_layout(self.model)
+
+    def _kernel_layout(self, m):
+        if self._mz is None:
+            return m
+        return m.index_select(m.dim() - 1, self._mz[0]).contiguous()
+
+    def set_model(self, model):
+        """Take a new model of the same shape (host array or device tensor)
+        in place; the tables keep pointing at it."""
+        m = torch.as_tensor(np.ascontiguousarray(model) if not isinstance(model, torch.Tensor) else model)
+        self.model.copy_(m.reshape(self.model.shape).to(device=self.dev, dtype=self.tdtype))
+        if self._mz is not None:
+            self.model_k.copy_(self._kernel_layout(self.model))
 
     def _injection_tables(self, entries):
         """Two-level CSR: tiles -> destination nodes -> entries.  Entries of
@@ -412,12 +443,17 @@ class DeviceBatch:
             ntx, nty, ntz = self.tiles_info[si][0]
             d.ntx, d.nty, d.ntz = ntx, nty, ntz
             d.off = self.offs[si]
-            d.model = self.model.data_ptr()
+            d.model = self.model_k.data_ptr()
             d.mny = self.model_shape[1] if self.dim == 3 else 1
-            d.mnz = self.model_shape[-1]
             wx = s.window[0]
             wz = s.window[-1]
             wy = s.window[1] if self.dim == 3 else (0, 0, 0)
+            if self._mz is not None:
+                # model rows already laid along the padded z axis
+                d.mnz = self._mz[1]
+                wz = (0, 0, self._mz[1] - 1)
+            else:
+                d.mnz = self.model_shape[-1]
             d.mx_off, d.mx_lo, d.mx_hi = wx
             d.my_off, d.my_lo, d.my_hi = wy
             d.mz_off, d.mz_lo, d.mz_hi = wz
@@ -642,6 +678,7 @@ class DeviceBatch:
         state = self._new_state()
         b = self._batch(state, False, dw=self.dw)
         b.inj_ptr = b.inj_node = b.inj_nptr = b.inj_chan = b.inj_w = None  # u1: Born source alone
+        m1 = self._kernel_layout(m1)
         b.born_m1 = m1.data_ptr()
         self._born_keep = m1
         self._run(b, 3, 0, self.max_nt, 1, 0)

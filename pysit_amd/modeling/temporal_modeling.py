@@ -86,14 +86,13 @@ class TemporalModeling(object):
         state and stores and only take the new model.  Per-shot calls never
         reuse, so device handles they return keep their own model and
         history."""
-        import torch
         s = self.solver
         key = (tuple(id(sh) for sh in shots), s.dt, s.nsteps, imaging_period, np.dtype(s.dtype).str,
                s.spatial_accuracy_order, tuple(s.mesh.shape(include_bc=True, as_grid=True)))
         cached = getattr(self, "_batch_cache", None)
         if reuse and cached is not None and cached[0] == key:
             b = cached[1]
-            b.model.copy_(torch.from_numpy(np.ascontiguousarray(s.unpadded_model(), dtype=b.dtype)))
+            b.set_model(s.unpadded_model())
             return b
         if reuse:
             self._batch_cache = None

@@ -59,7 +59,7 @@ class ShotProblem(object):
             self.batch = self.tm.make_batch([self.tm.geometry(sh) for sh in self.shots], C, self.imaging_period)
             self._key = key
         else:
-            self.batch.model.copy_(C.reshape(self.batch.model.shape))
+            self.batch.set_model(C)
         self.batch.timer = self.timer
         return self.batch
 
