@@ -56,7 +56,10 @@ def candidates(shots, dtype):
     rows = [t for t in (N.tile_rows(dim, code, 0), 4, 8) if N.tile_rows_supported(dim, code, t)]
     rows = list(dict.fromkeys(rows))
     alike = len({tuple(g.shape) for g in shots}) == 1 and len({g.nt for g in shots}) == 1
-    orders = ["shot", "interleave"] if alike and len(shots) > 1 else ["shot"]
+    # alike shots: the interleaved order first (the shots of a tile share
+    # its model rows in L2; measured 2-3 % ahead), so it is the default that
+    # the others must beat
+    orders = ["interleave", "shot"] if alike and len(shots) > 1 else ["shot"]
     return [{"tile_x": t, "tile_order": o} for t in rows for o in orders]
 
 

@@ -637,16 +637,21 @@ __device__ __forceinline__ void sample_group(const pysit_amd_batch &B, int slot,
     double acc = 0.0;
     if (live) {
         const T *u = (const T *)B.u[k & 1] + S->off;
-        for (int e = rd.z + g; e < rd.w; e += 16) acc += B.rec_w[e] * (double)u[B.rec_idx[e]];
+        acc = rec_gather<T>(B.rec_idx, B.rec_w, u, rd.z, rd.w, g);
     }
 #pragma unroll
     for (int m = 8; m >= 1; m >>= 1) acc += __shfl_xor(acc, m, 16);
     if (live && g == 0) S->trace[(long long)k * S->nrec + rd.y] = acc;
 }
 
-// Grid: the first `rec_blocks` blocks sample receivers (dispatched first, so
-// their short gather chains overlap the tiles), the rest own tiles.  SINGLE
-// is the reference-contract step: one shot passed by value, every tile PML.
+// Grid: the tile blocks, then `rec_blocks` blocks that sample receivers
+// (u_k, which no block of the launch writes).  Dispatched last, the short
+// receiver gathers fill the tail of the launch, where tile blocks leave CUs
+// idle (first, they cost 10 % of a forward step).  SINGLE is the
+// reference-contract step: one shot passed by value, every tile PML.
+#ifndef PA2D_REC_LAST
+#define PA2D_REC_LAST 1
+#endif
 #ifndef PA2D_WAVES_PER_EU
 #define PA2D_WAVES_PER_EU 1
 #endif
@@ -666,13 +671,19 @@ __global__ __launch_bounds__(256, PA2D_WAVES_PER_EU) void k_step2d(const pysit_a
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int b = blockIdx.x;
-    if (!is_adjoint(MODE) && b < rec_blocks) {
-        const int rw = b * 4 + w;
+    const int tile_blocks = gridDim.x - rec_blocks;
+#if PA2D_REC_LAST
+    const int rb = b - tile_blocks, tb = b;
+#else
+    const int rb = b, tb = b - rec_blocks;
+#endif
+    if (!is_adjoint(MODE) && rb >= 0 && rb < rec_blocks) {
+        const int rw = rb * 4 + w;
         const int rec = rw * 4 + (lane >> 4);
         if (rw * 4 < n_rec_lanes) sample_group<T>(B, slot, rec < n_rec_lanes ? rec : B.n_rec, lane & 15);
         return;
     }
-    const int gw = xcd_block(b - rec_blocks, gridDim.x - rec_blocks) * 4 + w;
+    const int gw = xcd_block(tb, tile_blocks) * 4 + w;
     if (gw >= n_tile_waves) return;
     if (SINGLE) {
         tile2d<T, TX, R, MODE, SINGLE>(B, P, P.shot0, gw, 1, gw, gw / P.shot0.ntz, slot, seg_base, lane, lds_f[w],

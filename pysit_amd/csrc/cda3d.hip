@@ -501,7 +501,7 @@ __device__ __forceinline__ void sample_group3(const pysit_amd_batch &B, int slot
     double acc = 0.0;
     if (live) {
         const T *u = (const T *)B.u[k & 1] + S->off;
-        for (int e = rd.z + g; e < rd.w; e += 16) acc += B.rec_w[e] * (double)u[B.rec_idx[e]];
+        acc = rec_gather<T>(B.rec_idx, B.rec_w, u, rd.z, rd.w, g);
     }
 #pragma unroll
     for (int m = 8; m >= 1; m >>= 1) acc += __shfl_xor(acc, m, 16);
@@ -524,13 +524,15 @@ void k_step3d(const pysit_amd_batch B, const Step3DParams<T> P,
         slot += cursor[0];
         seg_base = cursor[1];
     }
+    // the tile blocks, then the receiver blocks (they fill the launch's tail)
     const int b = blockIdx.x;
-    if (MODE != 2 && b < rec_blocks) {
-        const int rec = b * (TY3 * 4) + (threadIdx.x >> 4);
+    const int tile_blocks = gridDim.x - rec_blocks;
+    if (MODE != 2 && b >= tile_blocks) {
+        const int rec = (b - tile_blocks) * (TY3 * 4) + (threadIdx.x >> 4);
         sample_group3<T>(B, slot, rec < n_rec_lanes ? rec : B.n_rec, threadIdx.x & 15);
         return;
     }
-    const int blk = xcd_block(b - rec_blocks, gridDim.x - rec_blocks);
+    const int blk = xcd_block(b, tile_blocks);
     if (blk >= n_tile_blocks) return;
     if (P.single) {
         tile3d<T, R, MODE>(B, P, P.shot0, blk, 1, blk, slot, seg_base, L);

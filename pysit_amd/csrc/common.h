@@ -115,6 +115,35 @@ __host__ __device__ constexpr int z_last_tile(int nz, int align)
 {
     return nz > 64 ? ((nz - 64 + align - 1) / align) * align : 0;
 }
+// Receiver gather of one 16-lane group: lane g sums the entries g, g+16,
+// g+32, ... of [e0, e1) in that order, w[e] * u[idx[e]] in FP64.  The index
+// and weight loads of up to four entries go out before their field loads,
+// so a lane waits for two rounds of loads, not two per entry.
+template <typename T>
+__device__ __forceinline__ double rec_gather(const long long *__restrict__ idx, const double *__restrict__ w,
+                                             const T *__restrict__ u, int e0, int e1, int g)
+{
+    constexpr int U = 4;
+    double acc = 0.0;
+    for (int base = e0 + g; base < e1; base += 16 * U) {
+        long long ix[U];
+        double wv[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int e = base + 16 * q;
+            ix[q] = e < e1 ? idx[e] : 0;
+            wv[q] = e < e1 ? w[e] : 0.0;
+        }
+        T uv[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) uv[q] = u[ix[q]];
+#pragma unroll
+        for (int q = 0; q < U; ++q)
+            if (base + 16 * q < e1) acc += wv[q] * (double)uv[q];
+    }
+    return acc;
+}
+
 // row alignment in nodes: one 128-byte cache line
 template <typename T> constexpr int z_align() { return 128 / (int)sizeof(T); }
 
