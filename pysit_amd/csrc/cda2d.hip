@@ -248,6 +248,9 @@ __device__ __forceinline__ void ic_store(const C &c, int o, int xr, typename Acc
 #ifndef PA2D_PREFETCH_ADJ
 #define PA2D_PREFETCH_ADJ 2
 #endif
+#ifndef PA2D_PREFETCH_FULL
+#define PA2D_PREFETCH_FULL 2
+#endif
 // the dWaveOp history streams through (written once, read back thousands of
 // steps later): non-temporal accesses keep it from evicting the fields
 #ifndef PA2D_DW_NT
@@ -333,7 +336,7 @@ __device__ __forceinline__ void march_int(const Ctx2D<T, R, PACKED> &c, const St
 template <typename T, int TX, int R, int MODE, bool PACKED>
 __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const Step2DParams<T> &P, int x0)
 {
-    constexpr int D = is_adjoint(MODE) ? PA2D_PREFETCH_ADJ : PA2D_PREFETCH;
+    constexpr int D = PA2D_PREFETCH_FULL;
     constexpr int NU = TX + 2 * R;
     const int rb = c.rowb, nx = c.nx;
     const int so_u = c.s.so(F_U), so_uo = c.s.so(F_UO), so_px = c.s.so(F_PX), so_pz = c.s.so(F_PZ);
