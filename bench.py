@@ -202,10 +202,11 @@ def write_traffic(fetch_csv, write_csv, shots=16, shape=(1041, 341)):
                     "(bench.py --pmc-run, 40 slots)",
         "shots": shots, "grid_padded": list(shape), "tile_x": tx,
         "commands": [
-            "rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmcf -o run --output-format csv -- python3 bench.py "
-            "--pmc-run --steps 1 --warmup 0",
-            "rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmcw -o run --output-format csv -- python3 bench.py "
-            "--pmc-run --steps 1 --warmup 0",
+            "PYSIT_AMD_TUNE_CACHE=gpurun_out/tune.json python3 bench.py  (the launch configuration chosen)",
+            "PYSIT_AMD_TUNE_CACHE=gpurun_out/tune.json rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmcf -o run "
+            "--output-format csv -- python3 bench.py --pmc-run --steps 1 --warmup 0",
+            "PYSIT_AMD_TUNE_CACHE=gpurun_out/tune.json rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmcw -o run "
+            "--output-format csv -- python3 bench.py --pmc-run --steps 1 --warmup 0",
             "python3 bench.py --write-traffic gpurun_out/pmcf/run_counter_collection.csv "
             "gpurun_out/pmcw/run_counter_collection.csv"],
         "calibration": {"kernel": "torch.add(x, 1.0, out=y) over 2^26 float32 (reads and writes 262144 KiB)",
@@ -511,14 +512,14 @@ def main():
     v_true = model.C.reshape(nx, nz)
     v0 = gaussian_filter(v_true, sigma=300.0 / mesh.deltas[0], mode="nearest")
     v0 *= v_true.max() / v0.max()
-    tm = pa.TemporalModeling(solver, memory_budget=budget, tile_x=args.tile_x, autotune=not args.pmc_run)
+    tm = pa.TemporalModeling(solver, memory_budget=budget, tile_x=args.tile_x)
     d_obs = torch.from_numpy(np.stack(tm.forward_model_batch(shots, model))).to(dev)
     tm.last_batch = None
 
     m = torch.from_numpy(v0 ** -2.0).to(dev).requires_grad_(True)
-    # launch configuration: autotuned, except in the counter run, which keeps
-    # the default one (counter collection distorts the tuner's timings)
-    problem = ShotProblem(solver, shots, memory_budget=budget, tile_x=args.tile_x, autotune=not args.pmc_run)
+    # launch configuration: autotuned (a counter run should read the choice of
+    # a plain run from PYSIT_AMD_TUNE_CACHE: counters distort the tuner's timings)
+    problem = ShotProblem(solver, shots, memory_budget=budget, tile_x=args.tile_x)
 
     last = {}
 
@@ -601,7 +602,9 @@ def main():
     }
 
     if args.pmc_run:
-        print(json.dumps({"pmc_run": "done", "nt": nt, "shots": len(shots)}), flush=True)
+        b = problem.batch
+        print(json.dumps({"pmc_run": "done", "nt": nt, "shots": len(shots), "tile_x": b.tile_x,
+                          "tile_order": b.tile_order, "graph_steps": b.graph_steps}), flush=True)
         return
     if not (args.no_accuracy or args.profile_run):
         # the same gradient through the FP64 device path (matched to the FP64
