@@ -18,6 +18,8 @@ Besides the scipy operators (kept for API use) every point exposes its
 entries -- unpadded flat indices and weights in the reference's accumulation
 order -- which the device path turns into injection / sampling tables.
 """
+import copy
+
 import numpy as np
 import scipy.sparse as spsp
 
@@ -25,6 +27,18 @@ __all__ = ["MeshRepresentationBase", "PointRepresentationBase"]
 
 
 class MeshRepresentationBase(object):
+    # fixed once built (the mesh and the sampling operators and entries):
+    # copies share them, as a deep copy would reproduce them unchanged
+    _shared_on_copy = frozenset(("mesh", "domain", "deltas", "sampling_operator", "adjoint_sampling_operator",
+                                 "_sampling_operator_base", "_entry_index", "_entry_weight", "position"))
+
+    def __deepcopy__(self, memo):
+        new = self.__class__.__new__(self.__class__)
+        memo[id(self)] = new
+        for k, v in self.__dict__.items():
+            new.__dict__[k] = v if k in self._shared_on_copy else copy.deepcopy(v, memo)
+        return new
+
     def __init__(self, mesh, **kwargs):
         self.mesh = mesh
         self.domain = mesh.domain

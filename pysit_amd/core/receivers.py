@@ -169,26 +169,37 @@ def sampling_entries(receivers, padded_shape):
     return _entries(receivers, padded_shape, scale=True)
 
 
+def _gathered(receivers):
+    """Every receiver's entries in receiver order: (unpadded flat indices,
+    receiver ids, weights, prod(deltas) per entry), one mesh conversion for
+    all receivers that share the first one's mesh."""
+    recs = receivers._receivers()
+    idx, chan, w, pd = [], [], [], []
+    for r, rec in enumerate(recs):
+        i, wt = rec.entries()
+        idx.append(np.asarray(i, dtype=np.int64))
+        chan.append(np.full(len(idx[-1]), r, dtype=np.int64))
+        w.append(wt)
+        pd.append(np.full(len(idx[-1]), rec._prod_deltas))
+    return recs, np.concatenate(idx), np.concatenate(chan), np.concatenate(w), np.concatenate(pd)
+
+
+def _positions(recs, idx, chan):
+    mesh = recs[0].mesh
+    if all(rec.mesh is mesh for rec in recs):
+        return padded_positions(mesh, idx)
+    return np.concatenate([padded_positions(rec.mesh, idx[chan == r]) for r, rec in enumerate(recs)])
+
+
 def adjoint_entries(receivers):
     """Padded positions, receiver ids and weights of S* (no prod(deltas))."""
-    recs = receivers._receivers()
-    pos, chan, w = [], [], []
-    for r, rec in enumerate(recs):
-        idx, wt = rec.entries()
-        pos.append(padded_positions(rec.mesh, idx))
-        chan.append(np.full(idx.size, r, dtype=np.int64))
-        w.append(wt)
-    return np.concatenate(pos), np.concatenate(chan), np.concatenate(w)
+    recs, idx, chan, w, _ = _gathered(receivers)
+    return _positions(recs, idx, chan), chan, w
 
 
 def _entries(receivers, padded_shape, scale):
-    recs = receivers._receivers()
-    idx_all, chan, w = [], [], []
-    for r, rec in enumerate(recs):
-        idx, wt = rec.entries()
-        pos = padded_positions(rec.mesh, idx)
-        idx_all.append(np.ravel_multi_index(tuple(pos.T), tuple(padded_shape)))
-        chan.append(np.full(idx.size, r, dtype=np.int64))
-        # the sampling operator is the weight row times prod(deltas)
-        w.append(wt * rec._prod_deltas if scale else wt)
-    return np.concatenate(idx_all), np.concatenate(chan), np.concatenate(w)
+    recs, idx, chan, w, pd = _gathered(receivers)
+    pos = _positions(recs, idx, chan)
+    flat = np.ravel_multi_index(tuple(pos.T), tuple(padded_shape))
+    # the sampling operator is the weight row times prod(deltas)
+    return flat, chan, (w * pd if scale else w)

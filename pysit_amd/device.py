@@ -301,8 +301,13 @@ class DeviceBatch:
                     rows.append((si, t, int(fl[t]), t // per_x[si]))
         tiles = np.asarray(rows, dtype=np.int32).reshape(-1, 4)
         self.n_tiles = tiles.shape[0]
-        tile_index = {(int(r[0]), int(r[1])): i for i, r in enumerate(tiles)}
-        self._tile_index = tile_index
+        # global tile id of (shot, tile within the shot): one lookup array per shot
+        self._tile_lookup = []
+        for si in range(self.n_shots):
+            look = np.full(len(tiles_info[si][1]), -1, dtype=np.int64)
+            mine = tiles[:, 0] == si
+            look[tiles[mine, 1]] = np.nonzero(mine)[0]
+            self._tile_lookup.append(look)
         self.tiles = torch.from_numpy(tiles.reshape(-1).copy()).to(dev)
 
         # active prefix per slot
@@ -401,7 +406,7 @@ class DeviceBatch:
             if np.any(pos < 0) or np.any(pos >= np.asarray(s.shape)[None, :]):
                 raise ValueError("injection entry outside the padded grid")
             tloc, local = self._tile_of_pos(s, self.tiles_info[si][0], pos)
-            tglob = np.array([self._tile_index[(si, int(t))] for t in tloc], dtype=np.int64)
+            tglob = self._tile_lookup[si][np.asarray(tloc, dtype=np.int64)]
             node_tile.append(tglob)
             node_pos.append(local)
             e_chan.append(chan)

@@ -70,12 +70,21 @@ class TemporalModeling(object):
 
     # ---------------------------------------------------------------- plumbing
     def geometry(self, shot):
-        """The device description of ``shot`` for the current model."""
+        """The device description of ``shot`` for the current model.  The
+        receiver entries do not depend on the time axis: they are kept per
+        receiver set and grid."""
         s = self.solver
         shape = tuple(s.mesh.shape(include_bc=True, as_grid=True))
         src, amp = shot.sources.injection(s.dt, s.nsteps)
-        rec = sampling_entries(shot.receivers, shape)
-        adj = adjoint_entries(shot.receivers)
+        cache = self.__dict__.setdefault("_rec_entries", {})
+        key = (id(shot.receivers), shape)
+        hit = cache.get(key)
+        if hit is None or hit[0] is not shot.receivers:
+            if len(cache) > 4096:
+                cache.clear()
+            hit = (shot.receivers, sampling_entries(shot.receivers, shape), adjoint_entries(shot.receivers))
+            cache[key] = hit
+        rec, adj = hit[1], hit[2]
         return ShotGeometry(shape, s.kernel_pml(), s.model_window(), s.nsteps, src, amp, rec,
                             shot.receivers.receiver_count, adj=adj)
 
@@ -215,15 +224,28 @@ class TemporalModeling(object):
             adjointfield[:] = rv["adjointfield"][:]
         return rv["imaging_condition"].without_padding()
 
-    def gradient_batch(self, shots, m0, operands_fn, imaging_period=1):
+    def forward_traces_batch(self, shots, m0, imaging_period=1):
+        """Traces of many shots in one fused loop, as device FP64 tensors in
+        the shots' order (forward_model_batch without the host copies)."""
+        self.solver.model_parameters = m0
+        b = self.batch(shots, imaging_period, reuse=True)
+        traces = b.forward(keep_dw=False)
+        self.last_batch = None
+        return traces
+
+    def gradient_batch(self, shots, m0, operands_fn, imaging_period=1, device=False):
         """Forward all shots, form each shot's operand with ``operands_fn(i,
         simdata)``, migrate them together and return (sum of migrate_shot
-        results, list of operands)."""
+        results, list of operands).  With ``device`` the simdata handed to
+        ``operands_fn`` and the operands stay device tensors."""
         solver = self.solver
         solver.model_parameters = m0
         b = self.batch(shots, imaging_period, reuse=True)
         traces = b.forward(keep_dw=True)
-        ops = [np.asarray(operands_fn(i, t.cpu().numpy()), dtype=np.float64) for i, t in enumerate(traces)]
+        if device:
+            ops = [operands_fn(i, t) for i, t in enumerate(traces)]
+        else:
+            ops = [np.asarray(operands_fn(i, t.cpu().numpy()), dtype=np.float64) for i, t in enumerate(traces)]
         b.migrate(ops)
         g = self._fold_images(b, -solver.dt * imaging_period)
         self.last_batch = b

@@ -11,7 +11,10 @@ Mirrors pysit/objective_functions/temporal_least_squares.py:27-249:
 
 All shots of a call share one fused device loop per time step (the
 reference loops over shots); the per-shot results are identical to running
-``TemporalModeling`` shot by shot.  Shot-parallel MPI (the reference's
+``TemporalModeling`` shot by shot.  The batched evaluate / compute_gradient
+keep the traces and residuals on the device: the observed data are uploaded
+once per shot and interpolated onto each trial time axis there
+(:class:`ObservedOnDevice`).  Shot-parallel MPI (the reference's
 ``parallel_wrap_shot``) maps to one process per GPU; with a single process
 the wrap is a no-op.
 """
@@ -19,7 +22,58 @@ import numpy as np
 
 from ..modeling.temporal_modeling import TemporalModeling
 
-__all__ = ["TemporalLeastSquares"]
+__all__ = ["TemporalLeastSquares", "ObservedOnDevice"]
+
+
+class ObservedOnDevice(object):
+    """Observed traces of shots on the device, interpolated onto a time
+    axis as ``ReceiverBase.interpolate_data`` does on the host
+    (receivers.py:75-105): scipy's interp1d(kind='linear', bounds_error=False,
+    fill_value=0) restated with the same operations in the same order
+    (interp1d._call_linear: bracketing samples by searchsorted clipped to
+    [1, n-1], slope (y_hi - y_lo) / (x_hi - x_lo), value slope (t - x_lo) +
+    y_lo; zero outside [x_0, x_-1]).  Receivers with a time window take the
+    host code.  The data are uploaded once per shot (again when
+    ``receivers.data`` is replaced); the bracketing indices are kept per time
+    axis."""
+
+    def __init__(self):
+        self._data = {}
+        self._plans = {}
+
+    def __call__(self, shot, ts, dev):
+        import torch
+        rec = shot.receivers
+        if rec._time_window[0] not in ("None", "Set"):
+            # a time window multiplies the interpolated array as the host
+            # code does (receivers.py:95-96): take that path itself
+            return torch.as_tensor(np.asarray(rec.interpolate_data(ts), dtype=np.float64), device=dev)
+        d = rec.data
+        ent = self._data.get(id(shot))
+        if ent is None or ent[0] is not d or ent[1] is not shot:
+            ent = (d, shot, torch.as_tensor(np.ascontiguousarray(d, dtype=np.float64), device=dev))
+            self._data[id(shot)] = ent
+        y = ent[2]
+        x = np.asarray(rec.ts, dtype=np.float64)
+        ts = np.asarray(ts, dtype=np.float64)
+        key = (id(shot), x.shape[0], x[0], x[-1], ts.shape[0], ts[0], ts[-1])
+        plan = self._plans.get(key)
+        if plan is None:
+            hi = np.searchsorted(x, ts).clip(1, len(x) - 1)
+            lo = hi - 1
+            inside = ~((ts < x[0]) | (ts > x[-1]))
+            t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a, dtype=dt), device=dev)
+            plan = (t(lo, np.int64), t(hi, np.int64), t(x[lo], np.float64), t(x[hi], np.float64),
+                    t(ts, np.float64), t(inside, np.bool_))
+            if len(self._plans) > 4096:
+                self._plans.clear()
+            self._plans[key] = plan
+        lo, hi, x_lo, x_hi, tt, inside = plan
+        y_lo = y.index_select(0, lo)
+        y_hi = y.index_select(0, hi)
+        slope = (y_hi - y_lo) / (x_hi - x_lo)[:, None]
+        out = slope * (tt - x_lo)[:, None] + y_lo
+        return torch.where(inside[:, None], out, torch.zeros((), dtype=out.dtype, device=out.device))
 
 
 class _NullWrap(object):
@@ -37,6 +91,7 @@ class TemporalLeastSquares(object):
         self.parallel_wrap_shot = parallel_wrap_shot or _NullWrap()
         self.imaging_period = int(imaging_period)
         self.batch_size = batch_size
+        self.observed = ObservedOnDevice()
 
     def use_parallel(self):
         return self.parallel_wrap_shot.use_parallel
@@ -69,15 +124,16 @@ class TemporalLeastSquares(object):
         return out
 
     def evaluate(self, shots, m0, **kwargs):
-        r2 = 0.0
-        ts = None
+        import torch
+        r2 = None
         for group in self._groups(shots):
-            sims = self.modeling_tools.forward_model_batch(group, m0, self.imaging_period)
+            sims = self.modeling_tools.forward_traces_batch(group, m0, self.imaging_period)
             ts = self.solver.ts()
             for shot, sim in zip(group, sims):
-                r = shot.receivers.interpolate_data(ts) - sim
-                r2 += np.linalg.norm(r) ** 2
-        r2 = float(self._allreduce(np.array(r2)))
+                r = self.observed(shot, ts, sim.device) - sim
+                s2 = torch.sum(r * r)
+                r2 = s2 if r2 is None else r2 + s2
+        r2 = float(self._allreduce(np.array(0.0 if r2 is None else float(r2))))
         return 0.5 * r2 * self.solver.dt
 
     def _gradient_helper(self, shot, m0, ignore_minus=False, **kwargs):
@@ -91,16 +147,17 @@ class TemporalLeastSquares(object):
     def compute_gradient(self, shots, m0, aux_info={}, **kwargs):
         grad = m0.perturbation()
         r2 = 0.0
+        import torch
         for group in self._groups(shots):
             ts_holder = {}
 
             def residual(i, sim, group=group):
                 ts = ts_holder.setdefault("ts", self.solver.ts())
-                return group[i].receivers.interpolate_data(ts) - sim
+                return self.observed(group[i], ts, sim.device) - sim
 
-            g, rs = self.modeling_tools.gradient_batch(group, m0, residual, self.imaging_period)
+            g, rs = self.modeling_tools.gradient_batch(group, m0, residual, self.imaging_period, device=True)
             grad -= g
-            r2 += sum(float(np.linalg.norm(r) ** 2) for r in rs)
+            r2 += float(sum(torch.sum(r * r) for r in rs))
         if self.parallel_wrap_shot.use_parallel:
             r2 = float(self._allreduce(np.array(r2)))
             grad = m0.perturbation(data=self._allreduce(grad.asarray()))

@@ -85,3 +85,45 @@ def test_multiband_fwi_fp32_follows_fp64(gpu, fwi_runs):
     err = np.linalg.norm(c32 - c64) / np.linalg.norm(c64)
     print("final model FP32 vs FP64", err, flush=True)
     assert err < 1e-3
+
+
+def test_observed_on_device_matches_host_interpolation(gpu):
+    """ObservedOnDevice restates interp1d(kind='linear', fill_value=0): the
+    same values as the host path, on a finer and a longer time axis (zero
+    fill past the data)."""
+    import torch
+    import pysit_amd as pa
+    from pysit_amd.objective_functions.temporal_least_squares import ObservedOnDevice
+    mesh, solver, true_model, start_model = _small_fwi("double")
+    shots = _shots(mesh, 8.0, n=2)
+    pa.generate_seismic_data(shots, solver, true_model)
+    obs = ObservedOnDevice()
+    ts0 = solver.ts()
+    for ts in (ts0 * 0.97, np.arange(len(ts0) + 80) * solver.dt * 1.01, ts0):
+        for sh in shots:
+            dev = obs(sh, ts, gpu).cpu().numpy()
+            host = sh.receivers.interpolate_data(ts)
+            assert dev.shape == host.shape
+            assert np.max(np.abs(dev - host)) <= 1e-15 * np.max(np.abs(host))
+            assert np.all(dev[ts > ts0[-1]] == 0.0)
+
+
+def test_objective_matches_host_residual_formula(gpu):
+    """evaluate and the gradient's misfit, with residuals on the device,
+    equal 1/2 dt sum ||interp(d_obs) - S u||^2 formed on the host."""
+    import pysit_amd as pa
+    mesh, solver, true_model, start_model = _small_fwi("double")
+    shots = _shots(mesh, 8.0)
+    pa.generate_seismic_data(shots, solver, true_model)
+    solver.model_parameters = start_model
+    obj = pa.TemporalLeastSquares(solver)
+    f = obj.evaluate(shots, start_model)
+    aux = {"objective_value": (True, None)}
+    obj.compute_gradient(shots, start_model, aux_info=aux)
+    tm = pa.TemporalModeling(solver)
+    sims = tm.forward_model_batch(shots, start_model)
+    ts = solver.ts()
+    r2 = sum(np.linalg.norm(sh.receivers.interpolate_data(ts) - sim) ** 2 for sh, sim in zip(shots, sims))
+    ref = 0.5 * r2 * solver.dt
+    assert abs(f - ref) <= 1e-12 * ref
+    assert abs(aux["objective_value"][1] - ref) <= 1e-12 * ref
