@@ -127,3 +127,24 @@ def test_objective_matches_host_residual_formula(gpu):
     ref = 0.5 * r2 * solver.dt
     assert abs(f - ref) <= 1e-12 * ref
     assert abs(aux["objective_value"][1] - ref) <= 1e-12 * ref
+
+
+def test_supershot_traces_are_the_coded_sum(gpu):
+    """A source-encoded supershot is one device shot with a channel per
+    member source: its traces equal sum_i c_i d_i of the member shots'
+    traces (linearity), so its misfit vanishes at the true model, and its
+    gradient at another model is finite."""
+    import pysit_amd as pa
+    mesh, solver, true_model, start_model = _small_fwi("double")
+    shots = _shots(mesh, 8.0, n=4)
+    pa.generate_seismic_data(shots, solver, true_model)
+    sup = pa.SourceEncodedSupershot(shots, rng=np.random.default_rng(3))
+    tm = pa.TemporalModeling(solver)
+    tr = tm.forward_model(sup, true_model, return_parameters=["simdata"])["simdata"]
+    want = sup.receivers.data
+    assert np.linalg.norm(tr - want) <= 1e-12 * np.linalg.norm(want)
+    obj = pa.TemporalLeastSquares(solver)
+    f = obj.evaluate([sup], true_model)
+    assert f <= 1e-20 * 0.5 * solver.dt * np.linalg.norm(want) ** 2
+    g = obj.compute_gradient([sup], start_model)
+    assert np.all(np.isfinite(np.asarray(g.data))) and np.linalg.norm(np.asarray(g.data)) > 0

@@ -194,3 +194,52 @@ def test_plan_entry_points_check_arguments_without_a_device():
     stats = (ctypes.c_longlong * 4)()
     assert L.pysit_amd_plan_stats(None, ctypes.cast(stats, ctypes.c_void_p)) != 0
     assert L.pysit_amd_plan_destroy(None) == 0
+
+
+def _c1_shots_with_data(n=3, seed=0):
+    import pysit_amd as pa
+    m, shots, solver, model = c1_setup(order=4)
+    shots = pa.equispaced_acquisition(m, pa.RickerWavelet(10.0), sources=n, source_depth=0.05, receivers="max",
+                                      receiver_depth=0.05)
+    rng = np.random.default_rng(seed)
+    ts = np.arange(50) * 0.001
+    for sh in shots:
+        sh.receivers.reset_time_series(ts)
+        sh.receivers.data = rng.standard_normal((50, sh.receivers.receiver_count))
+    return m, shots
+
+
+def test_supershot_encodes_sources_and_data():
+    """SourceEncodedSupershot (shot.py:149-344): the encoded data are
+    sum_i c_i d_i, the encoded sources carry the codes as intensities, the
+    injection has one channel per member source scaled by its code."""
+    import pysit_amd as pa
+    m, shots = _c1_shots_with_data(3)
+    sup = pa.SourceEncodedSupershot(shots, rng=np.random.default_rng(5))
+    c = sup.codes
+    assert c.shape == (3,)
+    want = sum(ci * sh.receivers.data for ci, sh in zip(c, shots))
+    assert np.array_equal(sup.receivers.data, want)
+    assert np.array_equal(sup.receivers.ts, shots[0].receivers.ts)
+    assert [s.intensity for s in sup.sources.source_list] == list(c)
+    assert all(sh.sources.intensity == 1.0 for sh in shots)  # members untouched
+    (pos, chan, w), amp = sup.sources.injection(0.001, 40)
+    (_, _, _), amp0 = shots[1].sources.injection(0.001, 40)
+    assert amp.shape == (40, 3) and np.allclose(amp[:, 1], c[1] * amp0[:, 0], rtol=0, atol=1e-15)
+    sup.encode()
+    assert not np.array_equal(sup.codes, c)
+    k = pa.SourceEncodedSupershot(shots, weight_type="krebs", rng=np.random.default_rng(1))
+    assert set(np.unique(k.codes)) <= {-1.0, 1.0}
+
+
+def test_supershot_rejects_mixed_acquisition():
+    import pysit_amd as pa
+    m, shots = _c1_shots_with_data(2)
+    other = pa.equispaced_acquisition(m, pa.RickerWavelet(10.0), sources=1, source_depth=0.05, receivers=45,
+                                      receiver_depth=0.05)[0]
+    other.receivers.reset_time_series(shots[0].receivers.ts)
+    other.receivers.data = np.zeros((50, 45))
+    with pytest.raises(Exception, match="same for each shot"):
+        pa.SourceEncodedSupershot([shots[0], other])
+    with pytest.raises(Exception, match="weight type"):
+        pa.SourceEncodedSupershot(shots, weight_type="uniform")
