@@ -194,6 +194,17 @@ typedef struct {
      * holds the complete sum after each call.  NULL: every step adds into ic */
     float *ic32;
     int ic_window;
+    /* precomputed injection (optional, any mode): when inj_pre is not NULL,
+     * a run first evaluates the injection of every node of the CSR above for
+     * each of its slots, inj_pre[(slot - seg_base) * n_inj_nodes + n] (dtype
+     * of the batch, the same value the step would sum: FP64 sum in entry
+     * order, rounded once), and its steps read that value instead of summing
+     * the node's entries.  inj_node_shot[n] is the shot of node n; the
+     * buffer holds inj_pre_slots slots (slot_end - seg_base must fit).     */
+    void *inj_pre;
+    const int *inj_node_shot;
+    int n_inj_nodes;
+    int inj_pre_slots;
 } pysit_amd_batch;
 
 /* Run slots [slot_begin, slot_end) forward (direction = +1) or

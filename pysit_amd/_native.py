@@ -48,6 +48,7 @@ class Batch(ctypes.Structure):
         ("imaging_period", _i),
         ("fstore", _vp), ("born_m1", _vp),
         ("ic32", _vp), ("ic_window", _i),
+        ("inj_pre", _vp), ("inj_node_shot", _vp), ("n_inj_nodes", _i), ("inj_pre_slots", _i),
     ]
 
 

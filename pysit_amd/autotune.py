@@ -8,7 +8,7 @@ grids and the number of shots:
   more wavefronts and shorter marches, which suits small batches);
 * ``tile_order``: the tile table order, ``'shot'`` or, for shots of one
   shape and length, ``'interleave'``;
-* ``graph_steps``: the time loop as replayed hipGraphs of 32 steps, or one
+* ``graph_steps``: the time loop as replayed hipGraphs of GRAPH_STEPS (64) steps, or one
   launch per step.
 
 :func:`tuned_batch` builds the batch in each candidate configuration, times a
@@ -27,12 +27,11 @@ import os
 import numpy as np
 
 from . import _native as N
-from .device import DeviceBatch
+from .device import GRAPH_STEPS, DeviceBatch
 
 __all__ = ["tuned_batch", "signature", "candidates", "clear_cache"]
 
 _CACHE = {}
-GRAPH_STEPS = 32
 # a candidate replaces the one before it (the default first) only when it is
 # this much faster: near ties keep the default, so repeated runs choose alike
 MARGIN = 0.03

@@ -87,6 +87,42 @@ __global__ void k_fold(int nx, int ny, int nz, int lx, int rx, int ly, int ry, i
     *o = accumulate ? *o + scale * v : scale * v;
 }
 
+// ---- precomputed injection -------------------------------------------------
+// Every injection node's value for slots [slot_begin, slot_end): one thread
+// per node and kInjPreSlots slots, the same inj_sum the step kernels would
+// evaluate (zero where the node's shot is inactive), stored at
+// (slot - seg_base) * n_inj_nodes + n: consecutive threads, consecutive nodes.
+constexpr int kInjPreSlots = 16;
+template <typename T>
+__global__ void k_inj_pre(const pysit_amd_batch B, int slot_begin, int slot_end, int seg_base)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= B.n_inj_nodes) return;
+    const int s0 = slot_begin + (int)blockIdx.y * kInjPreSlots;
+    const int s1 = min(slot_end, s0 + kInjPreSlots);
+    const pysit_amd_shot &S = B.shots[B.inj_node_shot[n]];
+    const int e0 = B.inj_nptr[n], e1 = B.inj_nptr[n + 1];
+    T *out = (T *)B.inj_pre;
+    for (int slot = s0; slot < s1; ++slot) {
+        const int k = slot - S.kshift;
+        const double v = (k >= 0 && k < S.nt) ? inj_sum(B.inj_chan, B.inj_w, S.amp + (long long)k * S.namp, e0, e1) : 0.0;
+        out[(long long)(slot - seg_base) * B.n_inj_nodes + n] = (T)v;
+    }
+}
+
+static int inj_precompute(const pysit_amd_batch &B, int slot_begin, int slot_end, int seg_base, hipStream_t st)
+{
+    if (slot_end <= slot_begin || B.n_inj_nodes == 0) return PYSIT_AMD_OK;
+    const dim3 grid((unsigned)((B.n_inj_nodes + 255) / 256),
+                    (unsigned)((slot_end - slot_begin + kInjPreSlots - 1) / kInjPreSlots));
+    if (B.dtype)
+        hipLaunchKernelGGL(k_inj_pre<double>, grid, dim3(256), 0, st, B, slot_begin, slot_end, seg_base);
+    else
+        hipLaunchKernelGGL(k_inj_pre<float>, grid, dim3(256), 0, st, B, slot_begin, slot_end, seg_base);
+    PA_HIP(hipGetLastError());
+    return PYSIT_AMD_OK;
+}
+
 }  // namespace pa
 
 using namespace pa;
@@ -234,7 +270,15 @@ static int batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int slo
                      "active receiver count out of range");
         }
     }
+    PA_CHECK(!B.inj_pre || (B.inj_ptr && B.inj_node_shot && B.n_inj_nodes >= 0 && B.inj_pre_slots >= 0 &&
+                            slot_end - seg_base <= B.inj_pre_slots &&
+                            (slot_end - slot_begin + kInjPreSlots - 1) / kInjPreSlots <= 65535),
+             "precomputed injection: tables missing, or the run does not fit inj_pre_slots");
     hipStream_t st = (hipStream_t)stream;
+    if (B.inj_pre) {
+        const int rc = inj_precompute(B, slot_begin, slot_end, seg_base, st);
+        if (rc) return rc;
+    }
     if (B.dim == 2)
         return B.dtype ? run2d<double>(B, mode, slot_begin, slot_end, direction, seg_base,
                                        active_tiles, active_recs, plan, st)

@@ -586,10 +586,15 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
 #pragma unroll
             for (int j = 0; j < TX; ++j) lds_f[j * 64 + lane] = T(0);
             wave_sync();
-            for (int n = n0 + lane; n < n1; n += 64) {
-                double acc = 0.0;
-                for (int e = B.inj_nptr[n]; e < B.inj_nptr[n + 1]; ++e) acc += B.inj_w[e] * amp_k[B.inj_chan[e]];
-                lds_f[B.inj_node[n]] = (T)acc;
+            if (B.inj_pre) {
+                // values evaluated for the whole run before its first step
+                const T *pre = (const T *)B.inj_pre + (long long)(slot - seg_base) * B.n_inj_nodes;
+                for (int n = n0 + lane; n < n1; n += 64) lds_f[B.inj_node[n]] = pre[n];
+            } else {
+                for (int n = n0 + lane; n < n1; n += 64) {
+                    const int dst = B.inj_node[n];
+                    lds_f[dst] = (T)inj_sum(B.inj_chan, B.inj_w, amp_k, B.inj_nptr[n], B.inj_nptr[n + 1]);
+                }
             }
             wave_sync();
             c.has_inj = true;

@@ -472,10 +472,15 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
             const double *amp_k = S.amp + (long long)k * S.namp;
             for (int i = threadIdx.x; i < TX3 * TY3 * 64; i += TY3 * 64) L.f[i] = T(0);
             lds_barrier();
-            for (int n = n0 + (int)threadIdx.x; n < n1; n += TY3 * 64) {
-                double acc = 0.0;
-                for (int e = B.inj_nptr[n]; e < B.inj_nptr[n + 1]; ++e) acc += B.inj_w[e] * amp_k[B.inj_chan[e]];
-                L.f[B.inj_node[n]] = (T)acc;
+            if (B.inj_pre) {
+                // values evaluated for the whole run before its first step
+                const T *pre = (const T *)B.inj_pre + (long long)(slot - seg_base) * B.n_inj_nodes;
+                for (int n = n0 + (int)threadIdx.x; n < n1; n += TY3 * 64) L.f[B.inj_node[n]] = pre[n];
+            } else {
+                for (int n = n0 + (int)threadIdx.x; n < n1; n += TY3 * 64) {
+                    const int dst = B.inj_node[n];
+                    L.f[dst] = (T)inj_sum(B.inj_chan, B.inj_w, amp_k, B.inj_nptr[n], B.inj_nptr[n + 1]);
+                }
             }
             c.finj = &L.f[w * 64 + lane];
             c.fstride = TY3 * 64;

@@ -144,6 +144,36 @@ __device__ __forceinline__ double rec_gather(const long long *__restrict__ idx, 
     return acc;
 }
 
+// Injection value of one destination node: the sum over its entries
+// [e0, e1) of w[e] * amp_k[chan[e]] in entry order (FP64, each product
+// rounded before it is added).  The channel and weight loads of up to
+// kInjU entries go out together, then their amplitude loads, so a node
+// waits for two rounds of loads per kInjU entries instead of two per entry
+// (a Gaussian receiver spreads over 7 nodes per axis: 7 entries per node).
+constexpr int kInjU = 8;
+__device__ __forceinline__ double inj_sum(const int *__restrict__ chan, const double *__restrict__ w,
+                                          const double *__restrict__ amp_k, int e0, int e1)
+{
+#pragma clang fp contract(off)
+    double acc = 0.0;
+    for (int e = e0; e < e1; e += kInjU) {
+        int ch[kInjU];
+        double wv[kInjU], av[kInjU];
+#pragma unroll
+        for (int q = 0; q < kInjU; ++q) {
+            const bool ok = e + q < e1;
+            ch[q] = ok ? chan[e + q] : 0;
+            wv[q] = ok ? w[e + q] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < kInjU; ++q) av[q] = e + q < e1 ? amp_k[ch[q]] : 0.0;
+#pragma unroll
+        for (int q = 0; q < kInjU; ++q)
+            if (e + q < e1) acc += wv[q] * av[q];
+    }
+    return acc;
+}
+
 // row alignment in nodes: one 128-byte cache line
 template <typename T> constexpr int z_align() { return 128 / (int)sizeof(T); }
 

@@ -122,10 +122,18 @@ def _as_entries(e, dim):
 
 
 GRAPH_MAX_NODES = 1 << 20
+# FP32 imaging window (slots summed in FP32 before each FP64 flush) and the
+# default graph chunk, a multiple of it so that graph chunks hold whole windows
+IC_WINDOW = 64
+GRAPH_STEPS = 64
 
 
 class DeviceBatch:
     """Tables and state of a shot batch on one GPU."""
+
+    # adjoint sources evaluated per run before its first step (False: each
+    # step gathers them; the values are the same)
+    precompute_injection = True
 
     def __init__(self, shots, model, deltas, dt, order, dtype=np.float32, tile_x=None,
                  imaging_period=1, memory_budget=None, ic_window=None, tile_order="shot", graph_steps=None):
@@ -161,7 +169,7 @@ class DeviceBatch:
         # FP32 2D adjoints sum the imaging condition in FP32 over windows of
         # `ic_window` slots and in FP64 across windows (0: FP64 every step)
         if ic_window is None:
-            ic_window = 32 if (self.dim == 2 and self.dtype == np.float32) else 0
+            ic_window = IC_WINDOW if (self.dim == 2 and self.dtype == np.float32) else 0
         if ic_window and not (self.dim == 2 and self.dtype == np.float32):
             raise ValueError("the FP32 window accumulator is for 2D FP32 batches")
         self.ic_window = int(ic_window)
@@ -201,7 +209,7 @@ class DeviceBatch:
         self._build_tables()
         self._alloc_state()
         gs = self._graph_steps_arg
-        self.graph_steps = int(gs) if gs is not None else (32 if self.total_n < GRAPH_MAX_NODES else 0)
+        self.graph_steps = int(gs) if gs is not None else (GRAPH_STEPS if self.total_n < GRAPH_MAX_NODES else 0)
 
     # ------------------------------------------------------------------ tables
     def _tiles_of(self, s):
@@ -397,8 +405,9 @@ class DeviceBatch:
     def _injection_tables(self, entries):
         """Two-level CSR: tiles -> destination nodes -> entries.  Entries of
         one node keep the order they were given in (the order the reference's
-        sparse product accumulates in)."""
-        node_tile, node_pos, node_key, e_chan, e_w = [], [], [], [], []
+        sparse product accumulates in).  Also the shot of every node (for the
+        precomputed injection)."""
+        node_tile, node_pos, node_key, e_chan, e_w, e_shot = [], [], [], [], [], []
         for si, (s, e) in enumerate(zip(self.shots, entries)):
             pos, chan, w = e
             if pos.shape[0] == 0:
@@ -411,29 +420,34 @@ class DeviceBatch:
             node_pos.append(local)
             e_chan.append(chan)
             e_w.append(w)
+            e_shot.append(np.full(tglob.size, si, dtype=np.int32))
         if node_tile:
             tg = np.concatenate(node_tile)
             lp = np.concatenate(node_pos)
             ch = np.concatenate(e_chan)
             wt = np.concatenate(e_w)
+            es = np.concatenate(e_shot)
             order = np.lexsort((np.arange(tg.size), lp, tg))  # stable within a node
-            tg, lp, ch, wt = tg[order], lp[order], ch[order], wt[order]
+            tg, lp, ch, wt, es = tg[order], lp[order], ch[order], wt[order], es[order]
             key = tg * (1 << 20) + lp
             starts = np.concatenate([[True], key[1:] != key[:-1]])
             node_first = np.nonzero(starts)[0]
             n_tile = tg[node_first]
             n_pos = lp[node_first].astype(np.int32)
+            n_shot = es[node_first]
             nptr = np.concatenate([node_first, [tg.size]]).astype(np.int32)
             tcount = np.bincount(n_tile, minlength=self.n_tiles)
         else:
             n_pos = np.zeros(0, np.int32)
+            n_shot = np.zeros(0, np.int32)
             nptr = np.zeros(1, np.int32)
             ch = np.zeros(0, np.int64)
             wt = np.zeros(0)
             tcount = np.zeros(self.n_tiles, np.int64)
         ptr = np.concatenate([[0], np.cumsum(tcount)]).astype(np.int32)
         t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(self.dev)
-        return (t(ptr, np.int32), t(n_pos, np.int32), t(nptr, np.int32), t(ch, np.int32), t(wt, np.float64))
+        return (t(ptr, np.int32), t(n_pos, np.int32), t(nptr, np.int32), t(ch, np.int32), t(wt, np.float64),
+                t(n_shot, np.int32))
 
     def _shot_table(self, amps, namps):
         arr = (N.Shot * self.n_shots)()
@@ -479,7 +493,8 @@ class DeviceBatch:
             d.nrec = s.nrec
             d.trace = self.traces[si].data_ptr()
         raw = np.frombuffer(bytes(arr), dtype=np.uint8).copy()
-        return torch.from_numpy(raw).to(self.dev)
+        # pinned staging: the copy does not wait for the work queued before it
+        return torch.from_numpy(raw).pin_memory().to(self.dev, non_blocking=True)
 
     # ------------------------------------------------------------------- state
     def _fields(self):
@@ -518,7 +533,7 @@ class DeviceBatch:
         b.n_tiles = self.n_tiles
         b.tiles = self.tiles.data_ptr()
         inj = self.adj_inj if adjoint else self.fwd_inj
-        b.inj_ptr, b.inj_node, b.inj_nptr, b.inj_chan, b.inj_w = [x.data_ptr() for x in inj]
+        b.inj_ptr, b.inj_node, b.inj_nptr, b.inj_chan, b.inj_w = [x.data_ptr() for x in inj[:5]]
         b.n_rec = 0 if adjoint else self.n_rec
         b.recs = self.recs.data_ptr() if self.n_rec else None
         b.rec_idx = self.rec_idx.data_ptr() if self.n_rec else None
@@ -730,6 +745,16 @@ class DeviceBatch:
             qb.ic32 = ic32.data_ptr()
             qb.ic_window = self.ic_window
             self._ic32 = ic32
+        # the adjoint sources of each run (the whole loop, or one checkpoint
+        # segment) are evaluated before its first step: a step then reads one
+        # value per node instead of gathering the node's receiver entries
+        n_nodes = int(self.adj_inj[2].numel()) - 1
+        if n_nodes > 0 and self.precompute_injection:
+            slots = K if (self.ckpts and image) else self.max_nt
+            qb.inj_pre = self._inj_pre_buffer(slots, n_nodes).data_ptr()
+            qb.inj_node_shot = self.adj_inj[5].data_ptr()
+            qb.n_inj_nodes = n_nodes
+            qb.inj_pre_slots = slots
         self.adjoint_store = None
         if store_field:
             self.adjoint_store = self._field_store()
@@ -755,6 +780,16 @@ class DeviceBatch:
                 self._run(fb, 1, j0, j1, 1, j0, record=False)
                 self._run(qb, 2, j0, j1, -1, j0, record=False)
         return self.ic
+
+    def _inj_pre_buffer(self, slots, n_nodes):
+        """Device buffer for the precomputed injection of ``slots`` slots
+        (kept between calls)."""
+        need = slots * n_nodes
+        buf = getattr(self, "_inj_pre", None)
+        if buf is None or buf.numel() < need:
+            self._inj_pre = None
+            buf = self._inj_pre = torch.empty(need, dtype=self.tdtype, device=self.dev)
+        return buf
 
     def probe_step_time(self, slots=16, reps=2):
         """Seconds per slot of the forward step storing dWaveOp (mode 1 with
@@ -802,13 +837,28 @@ class DeviceBatch:
         """Padded FP64 image of one shot (caller order), a view of the accumulator."""
         return self._shot_view(self.ic, self.order_idx.index(si_caller))
 
+    def fold_sum(self, pads, out, scale=1.0):
+        """Add the fold of every shot's image onto ``out`` (the sum over the
+        shots of :meth:`fold`).  Shots of one padded shape are summed on the
+        padded grid first and folded once (the fold is linear)."""
+        s0 = self.shots[0]
+        if self.n_shots > 1 and all(s.shape == s0.shape for s in self.shots):
+            img = self.ic.view(self.n_shots, -1).sum(dim=0)
+            img = img.view(tuple(s0.shape[:-1]) + (self.zs[0],))[..., :s0.shape[-1]].contiguous()
+            self._fold_grid(img, s0.shape, pads, out, None, scale, True)
+        else:
+            for i in range(self.n_shots):
+                self.fold(i, pads, out, scale=scale, accumulate=True)
+
     def fold(self, si_caller, pads, out, out_origin=None, scale=1.0, accumulate=True):
         """Fold one shot's image onto ``out`` (device FP64 tensor holding the
         full unpadded model grid); ``out_origin`` is the shot window's first
         node in ``out`` (default 0)."""
         pos = self.order_idx.index(si_caller)
-        s = self.shots[pos]
         img = self._shot_view(self.ic, pos).contiguous()  # the fold reads a dense padded grid
+        self._fold_grid(img, self.shots[pos].shape, pads, out, out_origin, scale, accumulate)
+
+    def _fold_grid(self, img, shape, pads, out, out_origin, scale, accumulate):
         origin = out_origin or (0,) * self.dim
         if self.dim == 2:
             (lx, rx), (lz, rz) = pads
@@ -816,12 +866,12 @@ class DeviceBatch:
             onz = out.shape[-1]
             ony = 1
             o = out.reshape(-1)[origin[0] * onz + origin[1]:]
-            nx, nz = s.shape
+            nx, nz = shape
             ny = 1
         else:
             (lx, rx), (ly, ry), (lz, rz) = pads
             ony, onz = out.shape[1], out.shape[2]
             o = out.reshape(-1)[(origin[0] * ony + origin[1]) * onz + origin[2]:]
-            nx, ny, nz = s.shape
+            nx, ny, nz = shape
         N.call("pysit_amd_fold", self.dim, nx, ny, nz, lx, rx, ly, ry, lz, rz, _ptr(img), scale,
                _ptr(o), ony, onz, 1 if accumulate else 0, _stream())

@@ -63,7 +63,7 @@ class TemporalModeling(object):
         self.solver = solver
         self.memory_budget = memory_budget
         self.tile_x = tile_x
-        self.ic_window = ic_window  # None: the batch default (FP32 2D: 32 slots)
+        self.ic_window = ic_window  # None: the batch default (FP32 2D: IC_WINDOW = 64 slots)
         # launch configuration (tile rows, tile order, graphs) picked by timing
         # the candidates once per batch signature, unless tile_x is given
         self.autotune = bool(autotune)
@@ -133,8 +133,11 @@ class TemporalModeling(object):
         s = self.solver
         mesh = s.mesh
         out = torch.zeros(tuple(mesh.shape(as_grid=True)), dtype=torch.float64, device=batch.dev)
-        for i in (range(batch.n_shots) if shots_idx is None else shots_idx):
-            batch.fold(i, mesh.pad_widths(), out, scale=scale, accumulate=True)
+        if shots_idx is None:
+            batch.fold_sum(mesh.pad_widths(), out, scale=scale)
+        else:
+            for i in shots_idx:
+                batch.fold(i, mesh.pad_widths(), out, scale=scale, accumulate=True)
         return s.model_parameters.without_padding().perturbation(data=out.cpu().numpy().reshape(-1, 1))
 
     # ----------------------------------------------------------------- forward

@@ -40,6 +40,7 @@ class ShotProblem(object):
         self.segment = segment
         self.batch = None
         self._key = None
+        self._m_host = None
         self.timer = None
 
     def prepare(self, m_host):
@@ -50,8 +51,16 @@ class ShotProblem(object):
 
     def device_batch(self, m):
         """The batch for slowness squared ``m`` (device tensor): rebuilt when
-        dt or the step count change, else the model is updated in place."""
-        s = self.prepare(m.detach().cpu().numpy())
+        dt or the step count change, else the model is updated in place.
+        The solver takes the model only when it differs from the last one
+        this problem gave it and nothing else set a model since (the host
+        model rebuild is the costly part of a repeated call)."""
+        m_host = m.detach().cpu().numpy()
+        s = self.solver
+        if not (self._m_host is not None and self._m_host[1] == s._model_change_count
+                and np.array_equal(m_host, self._m_host[0])):
+            s = self.prepare(m_host)
+            self._m_host = (m_host.copy(), s._model_change_count)
         key = (s.dt, s.nsteps)
         C = (m.detach() ** -0.5).to(torch.float64)
         if self.batch is None or self._key != key:
@@ -82,8 +91,7 @@ class ShotTraces(torch.autograd.Function):
         b.migrate([grad_traces[i].contiguous() for i in range(b.n_shots)])
         out = torch.zeros(tuple(mesh.shape(as_grid=True)), dtype=torch.float64, device=b.dev)
         scale = -float(np.prod(mesh.deltas)) * p.imaging_period
-        for i in range(b.n_shots):
-            b.fold(i, mesh.pad_widths(), out, scale=scale, accumulate=True)
+        b.fold_sum(mesh.pad_widths(), out, scale=scale)
         return out.reshape(ctx.m_shape).to(grad_traces.dtype), None
 
 
