@@ -89,24 +89,61 @@ __global__ void k_fold(int nx, int ny, int nz, int lx, int rx, int ly, int ry, i
 
 // ---- precomputed injection -------------------------------------------------
 // Every injection node's value for slots [slot_begin, slot_end): one thread
-// per node and kInjPreSlots slots, the same inj_sum the step kernels would
-// evaluate (zero where the node's shot is inactive), stored at
-// (slot - seg_base) * n_inj_nodes + n: consecutive threads, consecutive nodes.
-constexpr int kInjPreSlots = 16;
+// per node and kInjPreSlots slots, the value inj_sum gives (zero where the
+// node's shot is inactive), stored at (slot - seg_base) * n_inj_nodes + n
+// (consecutive threads, consecutive nodes).  A node of at most kInjU entries
+// keeps its channels and weights in registers and loads the amplitudes of
+// kInjPreU slots together; the sum runs in entry order as in inj_sum.
+constexpr int kInjPreSlots = 32;
+constexpr int kInjPreU = 4;
 template <typename T>
 __global__ void k_inj_pre(const pysit_amd_batch B, int slot_begin, int slot_end, int seg_base)
 {
+#pragma clang fp contract(off)
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= B.n_inj_nodes) return;
     const int s0 = slot_begin + (int)blockIdx.y * kInjPreSlots;
     const int s1 = min(slot_end, s0 + kInjPreSlots);
     const pysit_amd_shot &S = B.shots[B.inj_node_shot[n]];
     const int e0 = B.inj_nptr[n], e1 = B.inj_nptr[n + 1];
-    T *out = (T *)B.inj_pre;
-    for (int slot = s0; slot < s1; ++slot) {
-        const int k = slot - S.kshift;
-        const double v = (k >= 0 && k < S.nt) ? inj_sum(B.inj_chan, B.inj_w, S.amp + (long long)k * S.namp, e0, e1) : 0.0;
-        out[(long long)(slot - seg_base) * B.n_inj_nodes + n] = (T)v;
+    const int ne = e1 - e0;
+    T *out = (T *)B.inj_pre + n;
+    const long long stride = B.n_inj_nodes;
+    if (ne > kInjU) {
+        for (int slot = s0; slot < s1; ++slot) {
+            const int k = slot - S.kshift;
+            const double v =
+                (k >= 0 && k < S.nt) ? inj_sum(B.inj_chan, B.inj_w, S.amp + (long long)k * S.namp, e0, e1) : 0.0;
+            out[(slot - seg_base) * stride] = (T)v;
+        }
+        return;
+    }
+    int ch[kInjU];
+    double wv[kInjU];
+#pragma unroll
+    for (int q = 0; q < kInjU; ++q) {
+        ch[q] = q < ne ? B.inj_chan[e0 + q] : 0;
+        wv[q] = q < ne ? B.inj_w[e0 + q] : 0.0;
+    }
+    for (int slot = s0; slot < s1; slot += kInjPreU) {
+        double av[kInjPreU][kInjU];
+        bool act[kInjPreU];
+#pragma unroll
+        for (int j = 0; j < kInjPreU; ++j) {
+            const int k = slot + j - S.kshift;
+            act[j] = slot + j < s1 && k >= 0 && k < S.nt;
+            const double *amp_k = S.amp + (long long)(act[j] ? k : 0) * S.namp;
+#pragma unroll
+            for (int q = 0; q < kInjU; ++q) av[j][q] = (act[j] && q < ne) ? amp_k[ch[q]] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < kInjPreU; ++j) {
+            double acc = 0.0;
+#pragma unroll
+            for (int q = 0; q < kInjU; ++q)
+                if (q < ne) acc += wv[q] * av[j][q];
+            if (slot + j < s1) out[(slot + j - seg_base) * stride] = (T)(act[j] ? acc : 0.0);
+        }
     }
 }
 
