@@ -124,12 +124,41 @@ def cpu_baseline(shape, pml_pairs, dt, deltas, seconds):
         list(ex.map(run, [steps] * threads))
         tm = time.perf_counter() - t0
     multi = n * steps * threads / tm / 1e9
+    # the reference's OpenMP variants (orders 4 and 6 only), one shot over all threads
+    omp = {}
+    if use_ref:
+        os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+        for order in (4, 6):
+            f = [rng.standard_normal(n).astype(np.float32) * 1e-3 for _ in range(6)]
+            f[4] = np.full(n, 3000.0, np.float32)
+
+            def omp_steps(k):
+                t0 = time.perf_counter()
+                for _ in range(k):
+                    px, pz, up = O.step2d(order, f[0], f[1], f[2], f[3], f[4], f[5], xl, xr, zl, zr, dt,
+                                          deltas[0], deltas[1], nx, nz, dtype=np.float32, use_ref=True,
+                                          fast=True, omp=True)
+                    f[0], f[3], f[1], f[2] = f[3], up, px, pz
+                return time.perf_counter() - t0
+            tp = omp_steps(3) / 3
+            k = max(3, int(0.5 * seconds / max(tp, 1e-6)))
+            omp[f"order{order}"] = n * k / omp_steps(k) / 1e9
+        omp["threads"] = int(os.environ["OMP_NUM_THREADS"])
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            cpu = next(l.split(":", 1)[1].strip() for l in fh if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     return {"value": multi, "unit": "Gpt-updates/s", "cores": threads, "kind": kind,
             "single_thread": single,
+            "openmp_variant": omp or None,
+            "host": {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "cpu": cpu},
             "sample": (f"{steps} steps of cda_time_scalar_2D<float,8> on the {nx}x{nz} padded grid per thread, "
                        f"{threads} threads (one shot each), -O3 -fopenmp -ffast-math; "
                        f"single thread {single:.4f} Gpt/s; the step-kernel rate bounds the CPU gradient rate "
-                       f"from above (imaging and dWaveOp work not counted)")}
+                       f"from above (imaging and dWaveOp work not counted); openmp_variant: the reference's "
+                       f"OpenMP step kernels of orders 4 and 6 (2D_4.h, 2D_6.h) on one shot over all threads")}
 
 
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",

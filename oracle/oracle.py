@@ -75,6 +75,11 @@ def ref_lib(fast=False):
     if key not in _LIBS:
         L = _load(os.path.join(HERE, "_ref", "libcda_ref_fast.so" if fast else "libcda_ref.so"))
         _declare(L, "ref")
+        for suf, P in (("d", _dp), ("f", _fp)):
+            f2 = getattr(L, f"ref_step2d_omp_{suf}", None)
+            if f2 is not None:
+                f2.restype = _i
+                f2.argtypes = ([_i] + [P] * 6 + [P, _i] * 4 + [_d] * 3 + [_i] * 2 + [P] * 3)
         _LIBS[key] = L
     return _LIBS[key]
 
@@ -98,9 +103,16 @@ def _arr(a, dtype):
 
 
 def step2d(order, km1_u, k_phix, k_phiz, k_u, C, rhs, xl, xr, zl, zr, dt, dx, dz, nx, nz,
-           dtype=np.float64, use_ref=False, fast=False):
-    """One 2D step; returns (kp1_phix, kp1_phiz, kp1_u) as flat arrays."""
+           dtype=np.float64, use_ref=False, fast=False, omp=False):
+    """One 2D step; returns (kp1_phix, kp1_phiz, kp1_u) as flat arrays.
+    ``omp`` (with ``use_ref``, orders 4 and 6): the reference's OpenMP
+    variant (constant_density_acoustic_time_scalar_2D_{4,6}.h), threaded over
+    OMP_NUM_THREADS (set here to the affinity count when unset)."""
     L = ref_lib(fast) if use_ref else lib(fast)
+    if omp:
+        if not use_ref or order not in (4, 6):
+            raise ValueError("the OpenMP variant is the reference's, orders 4 and 6")
+        os.environ.setdefault("OMP_NUM_THREADS", str(len(os.sched_getaffinity(0))))
     P = _dp if dtype == np.float64 else _fp
     suf = "d" if dtype == np.float64 else "f"
     ins = [_arr(a, dtype).reshape(-1) for a in (km1_u, k_phix, k_phiz, k_u, C, rhs)]
@@ -111,7 +123,8 @@ def step2d(order, km1_u, k_phix, k_phiz, k_u, C, rhs, xl, xr, zl, zr, dt, dx, dz
     for a in pml:
         args += [_ptr(a, P), a.size]
     args += [dt, dx, dz, nx, nz] + [_ptr(a, P) for a in outs]
-    rc = getattr(L, ("ref" if use_ref else "orc") + f"_step2d_{suf}")(*args)
+    name = ("ref" if use_ref else "orc") + ("_step2d_omp_" if omp else "_step2d_") + suf
+    rc = getattr(L, name)(*args)
     if rc != 0:
         raise ValueError(f"step2d failed ({rc})")
     return tuple(outs)
