@@ -148,7 +148,8 @@ typedef struct {
     const pysit_amd_shot *shots;
     /* tiles: int4 {shot, tile index t within the shot, flags, x tile index
      * t / (tiles per x row)}; flags bit0 = the tile is within `order/2` nodes
-     * of a PML node (full path).  Shots sorted so that, for every slot, the
+     * of a PML node (full path); 2D bit1 = no row of the tile or within
+     * `order/2` rows of it is an x PML row or an x edge row.  Shots sorted so that, for every slot, the
      * active tiles form a prefix.                                            */
     int n_tiles;
     const int *tiles;

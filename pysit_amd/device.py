@@ -230,7 +230,8 @@ class DeviceBatch:
                 for tz, (z0, zhi) in enumerate(zt):
                     z1 = zhi - 1
                     zp = (z0 - R < max(zl.size, 1)) or (z1 + R > nz - 1 - max(zr.size, 1))
-                    flags[tx * ntz + tz] = 1 if (xp or zp) else 0
+                    # bit0: the full path; bit1: its rows need no x PML form
+                    flags[tx * ntz + tz] = (1 if (xp or zp) else 0) | (0 if xp else 2)
             return (ntx, 1, ntz), flags
         nx, ny, nz = s.shape
         zt = z_tiles_2d(nz, self.zalign)
