@@ -86,6 +86,13 @@ struct Ctx3D {
     int hl;                  // LDS index (in a plane buffer) of this lane's z-halo slot in its own row, or -1
     int hrow;                // LDS row of this wave's y-halo row, or -1
     bool yzedge;             // y or z on the grid edge
+    // Phi (x, y, z) is nonzero only at PML nodes: a full tile reads and
+    // writes it only where the node lies in a PML band (every node for the
+    // reference-contract step, whose Phi input is arbitrary).  Lane masks:
+    // own node, y-halo row node, z-halo node in the y or z band; the x band
+    // is a per-row test.
+    bool gen, bp, bph, bpz;
+    int nxl, nxr;
     T syk, sym, szk, szm;    // lane y/z profiles (kernel, mesh)
     T vsig;                  // x profile of the tile planes: lane p < TX kernel, lane TX + p mesh
     const T *finj;
@@ -216,8 +223,9 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
         const int xx = x0 - R + (s_);                                                               \
         const bool in = xx >= 0 && xx < nx;                                                         \
         const int o = in ? at(co, xx * pl) : kOOB;                                                  \
+        const bool xb = c.gen || xx < c.nxl || xx >= nx - c.nxr;                                    \
         q[s_] = buf_ld<T>(c.u, o, 0);                                                               \
-        qx[s_] = buf_ld<T>(c.px, o, 0);                                                             \
+        qx[s_] = buf_ld<T>(c.px, (xb || c.bp) ? o : kOOB, 0);                                       \
         qs[s_] = buf_ld<T>(c.ps, o, 0);                                                             \
     } while (0)
 #define PA_ROW(r_)                                                                                  \
@@ -227,14 +235,15 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
         const int xs = in ? x : nx - 1, xp = xs * pl;                                              \
         const int oc = in ? at(co, xp) : kOOB, oz = in ? at(cho, xp) : kOOB;                        \
         const int oh = in ? at(hro, xp) : kOOB, oo = in ? at(ov, xp) : kOOB;                        \
+        const bool xb = c.gen || x < c.nxl || x >= nx - c.nxr;                                      \
         uz[r] = buf_ld<T>(c.u, oz, 0);                                                              \
         yh[r] = buf_ld<T>(c.u, oh, 0);                                                              \
         sz[r] = buf_ld<T>(c.ps, oz, 0);                                                             \
         sh[r] = buf_ld<T>(c.ps, oh, 0);                                                             \
-        pyc[r] = buf_ld<T>(c.py, oc, 0);                                                            \
-        pyh[r] = buf_ld<T>(c.py, oh, 0);                                                            \
-        pzc[r] = buf_ld<T>(c.pz, oc, 0);                                                            \
-        pzz[r] = buf_ld<T>(c.pz, oz, 0);                                                            \
+        pyc[r] = buf_ld<T>(c.py, (xb || c.bp) ? oc : kOOB, 0);                                      \
+        pyh[r] = buf_ld<T>(c.py, (xb || c.bph) ? oh : kOOB, 0);                                     \
+        pzc[r] = buf_ld<T>(c.pz, (xb || c.bp) ? oc : kOOB, 0);                                      \
+        pzz[r] = buf_ld<T>(c.pz, (xb || c.bpz) ? oz : kOOB, 0);                                     \
         um[r] = buf_ld<T>(c.uo, oo, 0);                                                             \
         const int mp = clampi(xs - c.mx_off, c.mx_lo, c.mx_hi) * c.mplane;                         \
         cv[r] = buf_ld<T>(c.model, mv, mp);                                                         \
@@ -343,11 +352,13 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
             psn = T(0);
         }
         const int oo = inx ? at(ov, xp) : kOOB;
+        // Phi keeps its value (zero) where the node is not a PML node
+        const int op = (c.gen || pml || edge) ? oo : kOOB;
         if (MODE == 2) buf_st(c.ic, inx ? at(ovi, xp * (8 / SZ)) : kOOB, c.shb8, icv[j] + (double)uc * (double)dv[j]);
         buf_st(c.uo, oo, 0, up);
-        buf_st(c.pxo, oo, 0, pxn);
-        buf_st(c.pyo, oo, 0, pyn);
-        buf_st(c.pzo, oo, 0, pzn);
+        buf_st(c.pxo, op, 0, pxn);
+        buf_st(c.pyo, op, 0, pyn);
+        buf_st(c.pzo, op, 0, pzn);
         buf_st(c.pso, oo, 0, psn);
         if (MODE == 1) buf_st(c.dw, inx ? at(ovd, xp) : kOOB, c.shb, dw);
         if (c.has_fst) buf_st(c.fst, inx ? at(ovf, xp) : kOOB, c.shb, uc);
@@ -401,6 +412,16 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
     const int hy = y0 - R + c.hrow;
     c.hro = (c.hrow >= 0 && hy >= 0 && hy < ny && zin) ? (hy * zs + z) * SZ : kOOB;
     c.yzedge = y == 0 || y >= ny - 1 || z == 0 || z >= nz - 1;
+    {
+        auto yb = [&](int yy) { return yy < S.nyl || yy >= ny - S.nyr; };
+        auto zb = [&](int zz) { return zz < S.nzl || zz >= nz - S.nzr; };
+        c.gen = P.single != 0;
+        c.bp = yb(y) || zb(z);
+        c.bph = yb(hy) || zb(z);
+        c.bpz = yb(y) || zb(hz);
+        c.nxl = S.nxl;
+        c.nxr = S.nxr;
+    }
     const int taken = (MODE == 2) ? (S.nt - 1 - k) : k;
     const int cur = taken & 1;
     const long long off = S.off;
