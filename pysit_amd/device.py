@@ -572,17 +572,24 @@ class DeviceBatch:
     def plan_segments(self, keep_dw):
         """Slots per dWaveOp segment K (K = max_nt means every step is stored).
 
-        The store needs K * total_n values; each checkpoint keeps the whole
+        The store needs K * total_n values, the adjoint's precomputed
+        sources K values per injection node; each checkpoint keeps the whole
         state.  K is the largest value whose store plus checkpoints fit the
-        memory budget (default: 80% of the free device memory)."""
+        memory budget (default: 90% of the memory available to this batch:
+        free device memory, memory torch holds unused, and this batch's own
+        store and checkpoints, which a new run reuses or releases)."""
         if not keep_dw:
             return self.max_nt
         budget = self.memory_budget
         if budget is None:
             free, _ = torch.cuda.mem_get_info(self.dev)
-            budget = int(0.8 * free)
+            unused = torch.cuda.memory_reserved(self.dev) - torch.cuda.memory_allocated(self.dev)
+            held = self.dw.numel() * self.dw.element_size() if getattr(self, "dw", None) is not None else 0
+            held += sum(v.numel() * v.element_size() for c in getattr(self, "ckpts", []) for v in c.values())
+            budget = int(0.9 * (free + unused + held))
         item = self.dtype.itemsize
-        per_slot = self.total_n * item
+        n_inj = int(self.adj_inj[2].numel()) - 1 if self.adj_inj is not None else 0
+        per_slot = (self.total_n + n_inj) * item
         ckpt = self.state_bytes()
         if self.max_nt * per_slot <= budget:
             return self.max_nt
