@@ -26,6 +26,15 @@ import scipy.sparse as spsp
 __all__ = ["MeshRepresentationBase", "PointRepresentationBase"]
 
 
+_SCALARS = (type(None), bool, int, float, complex, str, np.generic)
+
+
+def _immutable(v):
+    """Values a deep copy would return unchanged: scalars, strings, None,
+    and tuples of them."""
+    return isinstance(v, _SCALARS) or (type(v) is tuple and all(isinstance(x, _SCALARS) for x in v))
+
+
 class MeshRepresentationBase(object):
     # fixed once built (the mesh and the sampling operators and entries):
     # copies share them, as a deep copy would reproduce them unchanged
@@ -36,7 +45,10 @@ class MeshRepresentationBase(object):
         new = self.__class__.__new__(self.__class__)
         memo[id(self)] = new
         for k, v in self.__dict__.items():
-            new.__dict__[k] = v if k in self._shared_on_copy else copy.deepcopy(v, memo)
+            if k in self._shared_on_copy or _immutable(v):
+                new.__dict__[k] = v
+            else:
+                new.__dict__[k] = copy.deepcopy(v, memo)
         return new
 
     def __init__(self, mesh, **kwargs):

@@ -77,9 +77,22 @@ class DerivativeGaussianPulse(SourceWaveletBase):
         v[np.abs(v) < self.threshold] = 0.0
         return v[0] if scalar else v
 
+    _amp_cache = {}
+
     def amplitudes(self, dt, nsteps):
-        """w(k*dt) for k = 0..nsteps-1, each time value formed as k*dt."""
-        return np.array([self._evaluate_time(k * dt) for k in range(nsteps)], dtype=np.float64)
+        """w(k*dt) for k = 0..nsteps-1, each time value formed as k*dt and
+        evaluated as a scalar, as the reference's time loop does; the array is
+        kept per (wavelet, dt, nsteps), so the shots of a survey evaluate it
+        once (a read-only array)."""
+        key = (type(self), self.order, self.sigma, self.t_shift, self.threshold, float(dt), int(nsteps))
+        a = DerivativeGaussianPulse._amp_cache.get(key)
+        if a is None:
+            a = np.array([self._evaluate_time(k * dt) for k in range(nsteps)], dtype=np.float64)
+            a.setflags(write=False)
+            if len(DerivativeGaussianPulse._amp_cache) >= 64:  # time axes change as dt follows the model
+                DerivativeGaussianPulse._amp_cache.clear()
+            DerivativeGaussianPulse._amp_cache[key] = a
+        return a
 
 
 class RickerWavelet(DerivativeGaussianPulse):

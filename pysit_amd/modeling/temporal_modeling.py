@@ -72,17 +72,20 @@ class TemporalModeling(object):
     def geometry(self, shot):
         """The device description of ``shot`` for the current model.  The
         receiver entries do not depend on the time axis: they are kept per
-        receiver set and grid."""
+        sampling operator and grid, so the copies of one receiver set (a
+        survey's shots share its operator) gather them once."""
         s = self.solver
         shape = tuple(s.mesh.shape(include_bc=True, as_grid=True))
         src, amp = shot.sources.injection(s.dt, s.nsteps)
         cache = self.__dict__.setdefault("_rec_entries", {})
-        key = (id(shot.receivers), shape)
+        op = shot.receivers.sampling_operator
+        op = shot.receivers if op is None else op
+        key = (id(op), shape)
         hit = cache.get(key)
-        if hit is None or hit[0] is not shot.receivers:
+        if hit is None or hit[0] is not op:
             if len(cache) > 4096:
                 cache.clear()
-            hit = (shot.receivers, sampling_entries(shot.receivers, shape), adjoint_entries(shot.receivers))
+            hit = (op, sampling_entries(shot.receivers, shape), adjoint_entries(shot.receivers))
             cache[key] = hit
         rec, adj = hit[1], hit[2]
         return ShotGeometry(shape, s.kernel_pml(), s.model_window(), s.nsteps, src, amp, rec,
