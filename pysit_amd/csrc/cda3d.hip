@@ -205,7 +205,10 @@ __device__ __forceinline__ void march3_int(const Ctx3D<T, R> &c, const Step3DPar
 }
 
 // ---- PML tiles -----------------------------------------------------------------
-template <typename T, int R, int MODE>
+// XC: no plane of the tile or within R planes of it is an x PML plane or an
+// x edge (the host flags such tiles): sigma_x is zero on every plane, the
+// terms it multiplies drop out and the damping factor is one per lane.
+template <typename T, int R, int MODE, bool XC>
 __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DParams<T> &P, Lds3D<T, R> &L,
                                             int x0, int hl_pz)
 {
@@ -302,7 +305,7 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
         const T lap = (lx + lyy) + lz;
         const T c2 = cv[j] * cv[j];
         const T u_m = um[j];
-        const bool edge = (x == 0) | (x == nx - 1) | c.yzedge;
+        const bool edge = XC ? c.yzedge : ((x == 0) | (x == nx - 1) | c.yzedge);
         T dux = T(0), duy = T(0), duz = T(0), dpx = T(0), dpy = T(0), dpz = T(0);
         T dsx = T(0), dsy = T(0), dsz = T(0);
 #pragma unroll
@@ -319,30 +322,51 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
             dsz = fma(az, lps[own + i] - lps[own - i], dsz);
         }
         const T src = (((f + lap) + dpx) + dpy) + dpz;
-        const T sxk = bcast3(vsig, j), sxm = bcast3(vsig, TX + j);
         const T dt = P.tc.dt;
         T psn = psc + dt * uc;
-        const T s1k = (sxk + syk) + szk;
-        const T s2k = (sxk * syk + syk * szk) + szk * sxk;
-        const T s3k = (sxk * syk) * szk;
-        const bool pml = sxk != T(0) || syk != T(0) || szk != T(0);
         const T pxc = qx[R + j], pycj = pyc[j], pzcj = pzc[j];
-        const T pxd = (pxc - dt * sxk * pxc) + dt * ((syk + szk) - sxk) * dux + dt * (syk * szk * dsx);
-        const T pyd = (pycj - dt * syk * pycj) + dt * ((szk + sxk) - syk) * duy + dt * (szk * sxk * dsy);
-        const T pzd = (pzcj - dt * szk * pzcj) + dt * ((sxk + syk) - szk) * duz + dt * (sxk * syk * dsz);
+        T s1k, s2k, s1m, s2m, srcp, pxd, pyd, pzd, dws;
+        bool pml;
+        if constexpr (XC) {
+            // sigma_x = 0: the form below with its zero terms dropped (the
+            // same values; at most the sign of a zero differs), and every
+            // factor of the lane's own profiles, the division included, the
+            // same on every plane
+            s1k = syk + szk;
+            s2k = syk * szk;
+            s1m = sym + szm;
+            s2m = sym * szm;
+            pml = syk != T(0) || szk != T(0);
+            srcp = src;
+            pxd = pxc + dt * (syk + szk) * dux + dt * (syk * szk * dsx);
+            pyd = (pycj - dt * syk * pycj) + dt * (szk - syk) * duy;
+            pzd = (pzcj - dt * szk * pzcj) + dt * (syk - szk) * duz;
+            dws = T(0);
+        } else {
+            const T sxk = bcast3(vsig, j), sxm = bcast3(vsig, TX + j);
+            s1k = (sxk + syk) + szk;
+            s2k = (sxk * syk + syk * szk) + szk * sxk;
+            const T s3k = (sxk * syk) * szk;
+            pml = sxk != T(0) || syk != T(0) || szk != T(0);
+            srcp = src - s3k * psc;
+            pxd = (pxc - dt * sxk * pxc) + dt * ((syk + szk) - sxk) * dux + dt * (syk * szk * dsx);
+            pyd = (pycj - dt * syk * pycj) + dt * ((szk + sxk) - syk) * duy + dt * (szk * sxk * dsy);
+            pzd = (pzcj - dt * szk * pzcj) + dt * ((sxk + syk) - szk) * duz + dt * (sxk * syk * dsz);
+            s1m = (sxm + sym) + szm;
+            s2m = (sxm * sym + sxm * szm) + sym * szm;
+            const T s3m = (sxm * sym) * szm;
+            dws = s3m * psc;
+        }
         const T fac1 = P.tc.two_dt2 / (T(2) + dt * s1k);
-        const T fac2 = c2 * (src - s3k * psc) - (u_m - T(2) * uc) * P.tc.inv_dt2 + s1k * u_m * P.tc.inv_2dt -
-                       s2k * uc;
+        const T fac2 = c2 * srcp - (u_m - T(2) * uc) * P.tc.inv_dt2 + s1k * u_m * P.tc.inv_2dt - s2k * uc;
         const T upd = fac1 * fac2;
         const T upi = fma(P.tc.dt2, c2 * src, T(2) * uc - u_m);
         T pxn = pml ? pxd : pxc;
         T pyn = pml ? pyd : pycj;
         T pzn = pml ? pzd : pzcj;
         T up = pml ? upd : upi;
-        const T s1m = (sxm + sym) + szm;
-        const T s2m = (sxm * sym + sxm * szm) + sym * szm;
-        const T s3m = (sxm * sym) * szm;
-        T dw = c2 * (src - s3k * psc) + (s1m - s1k) * (up - u_m) * P.tc.inv_2dt + (s2m - s2k) * uc + s3m * psc;
+        T dw = c2 * srcp + (s1m - s1k) * (up - u_m) * P.tc.inv_2dt + (s2m - s2k) * uc;
+        if constexpr (!XC) dw = dw + dws;
         if (edge) {
             up = T(0);
             dw = T(0);
@@ -369,7 +393,7 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
 
 template <typename T, int R, int MODE>
 __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DParams<T> &P,
-                                       const pysit_amd_shot &S, int tile, int full, int tloc, int slot,
+                                       const pysit_amd_shot &S, int tile, int full, int xclear, int tloc, int slot,
                                        int seg_base, Lds3D<T, R> &L)
 {
     constexpr int SZ = sizeof(T), TX = TX3, LZ = Lds3D<T, R>::LZ;
@@ -509,7 +533,8 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
     }
     if (!has_inj && threadIdx.x == 0) L.junk[3][0] = T(0);
     lds_barrier();
-    if (full) march3_full<T, R, MODE>(c, P, L, x0, hl_pz);
+    if (full && xclear) march3_full<T, R, MODE, true>(c, P, L, x0, hl_pz);
+    else if (full) march3_full<T, R, MODE, false>(c, P, L, x0, hl_pz);
     else march3_int<T, R, MODE>(c, P, L, x0);
 }
 
@@ -561,10 +586,10 @@ void k_step3d(const pysit_amd_batch B, const Step3DParams<T> P,
     const int blk = xcd_block(b, tile_blocks);
     if (blk >= n_tile_blocks) return;
     if (P.single) {
-        tile3d<T, R, MODE>(B, P, P.shot0, blk, 1, blk, slot, seg_base, L);
+        tile3d<T, R, MODE>(B, P, P.shot0, blk, 1, 0, blk, slot, seg_base, L);
     } else {
         const int4 td = reinterpret_cast<const int4 *>(B.tiles)[blk];
-        tile3d<T, R, MODE>(B, P, B.shots[td.x], blk, td.z & 1, td.y, slot, seg_base, L);
+        tile3d<T, R, MODE>(B, P, B.shots[td.x], blk, td.z & 1, (td.z >> 1) & 1, td.y, slot, seg_base, L);
     }
 }
 

@@ -248,7 +248,8 @@ class DeviceBatch:
                 yp = (y0 - R < max(yl.size, 1)) or (y1 + R > ny - 1 - max(yr.size, 1))
                 for tz, (z0, zhi) in enumerate(zt):
                     zp = (z0 - R < max(zl.size, 1)) or (zhi - 1 + R > nz - 1 - max(zr.size, 1))
-                    flags[(tx * nty + ty) * ntz + tz] = 1 if (xp or yp or zp) else 0
+                    # bit0: the full path; bit1: its planes need no x PML form
+                    flags[(tx * nty + ty) * ntz + tz] = (1 if (xp or yp or zp) else 0) | (0 if xp else 2)
         return (ntx, nty, ntz), flags
 
     def _tile_of_pos(self, s, s_tiles, pos):

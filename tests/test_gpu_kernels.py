@@ -234,7 +234,7 @@ def test_batch_3d_with_plain_tiles(gpu, dtype, tol):
                   src_center=(16, 18, 92), rec_z=96)
     b = _batch([p], dtype)
     flags = b.tiles.view(-1, 4)[:, 2].cpu().numpy()
-    assert (flags == 0).sum() >= 4
+    assert ((flags & 1) == 0).sum() >= 4  # bit0: the PML path
     tr = b.forward(keep_dw=True, store_field=True)[0].cpu().numpy()
     field = b.shot_field(b.field_store, 0, p.nt - 1).cpu().numpy().reshape(-1)
     op = p.oracle_problem()
