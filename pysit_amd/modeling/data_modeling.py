@@ -21,7 +21,7 @@ Shot data on disk (:62-131 save, ``generate_seismic_data_from_file``
 """
 import os
 import pickle
-import shutil
+import re
 import time
 
 import numpy as np
@@ -63,15 +63,21 @@ def _paths(path, i, method):
 
 
 def save_shot_data(shots, save_method, path="./shots_time"):
-    """Write every shot's traces and time axis (the directory is replaced,
-    as the reference does)."""
+    """Write every shot's traces and time axis.  The shot files of an
+    earlier save in ``path`` (``shot_<i>`` / ``ts_<i>`` in any of the formats)
+    are removed first, so a load finds this save only; the reference removes
+    its fixed ``./shots_time`` directory whole, which here would also take
+    other files of a directory the caller names."""
     if save_method not in _EXT:
         raise TypeError("Unknown save_method")
     if save_method == "h5py":
         raise ImportError("h5py is not installed please install it and try again")
-    if os.path.exists(path):
-        shutil.rmtree(path)
-    os.makedirs(path)
+    if os.path.isdir(path):
+        stale = re.compile(r"^(shot|ts)_[0-9]+(" + "|".join(re.escape(e) for e in _EXT.values()) + r")$")
+        for name in os.listdir(path):
+            if stale.match(name) and os.path.isfile(os.path.join(path, name)):
+                os.remove(os.path.join(path, name))
+    os.makedirs(path, exist_ok=True)
     for i, shot in enumerate(shots):
         fshot, fts = _paths(path, i, save_method)
         data, ts = np.asarray(shot.receivers.data), np.asarray(shot.receivers.ts)

@@ -249,19 +249,22 @@ def test_supershot_rejects_mixed_acquisition():
 def test_shot_data_files_round_trip(method, tmp_path, monkeypatch):
     """Shot data on disk (data_modeling.py:62-131 save, :180-330 load): each
     shot's traces and time axis come back bit for bit into fresh shots'
-    receivers, under ./shots_time/shot_<i>, ts_<i>; a second save replaces
-    the directory."""
+    receivers, under ./shots_time/shot_<i>, ts_<i>; a save removes the shot
+    files of an earlier one and leaves other files alone."""
     import pysit_amd as pa
     from pysit_amd.modeling.data_modeling import _EXT
     monkeypatch.chdir(tmp_path)
     m, shots = _c1_shots_with_data(3, seed=1)
     solver = pa.ConstantDensityAcousticWave(m, spatial_accuracy_order=4)
     (tmp_path / "shots_time").mkdir()
-    (tmp_path / "shots_time" / "stale.txt").write_text("x")
+    (tmp_path / "shots_time" / "notes.txt").write_text("x")
+    for old in ("shot_7.mat", "ts_7.mat", "shot_4.save", "ts_9.npy"):
+        (tmp_path / "shots_time" / old).write_text("stale")
     pa.save_shot_data(shots, method)
     names = sorted(os.listdir(tmp_path / "shots_time"))
     ext = _EXT[method]
-    assert names == sorted([f"shot_{i}{ext}" for i in (1, 2, 3)] + [f"ts_{i}{ext}" for i in (1, 2, 3)])
+    assert names == sorted([f"shot_{i}{ext}" for i in (1, 2, 3)] + [f"ts_{i}{ext}" for i in (1, 2, 3)] +
+                           ["notes.txt"])
     _, fresh = _c1_shots_with_data(3, seed=2)
     pa.generate_seismic_data_from_file(fresh, solver, save_method=method)
     for a, b in zip(shots, fresh):
