@@ -150,17 +150,19 @@ def test_supershot_traces_are_the_coded_sum(gpu):
     assert np.all(np.isfinite(np.asarray(g.data))) and np.linalg.norm(np.asarray(g.data)) > 0
 
 
-def test_generated_data_saved_and_loaded(gpu, tmp_path, monkeypatch):
-    """generate_seismic_data(..., save_method='savemat') writes the traces the
-    device produced; generate_seismic_data_from_file puts them on fresh
-    shots, and the objective on those equals the objective on the originals."""
+@pytest.mark.parametrize("method", ["savemat", "pickle", "npy"])
+def test_generated_data_saved_and_loaded(gpu, tmp_path, monkeypatch, method):
+    """As the reference's examples/test_save.py: generate_seismic_data(...,
+    save_method=...) writes the traces the device produced;
+    generate_seismic_data_from_file puts them on fresh shots bit for bit,
+    and the objective on those equals the objective on the originals."""
     import pysit_amd as pa
     monkeypatch.chdir(tmp_path)
     mesh, solver, true_model, start_model = _small_fwi("double")
     shots = _shots(mesh, 8.0, n=2)
-    pa.generate_seismic_data(shots, solver, true_model, save_method="savemat")
+    pa.generate_seismic_data(shots, solver, true_model, save_method=method)
     fresh = _shots(mesh, 8.0, n=2)
-    pa.generate_seismic_data_from_file(fresh, solver, save_method="savemat")
+    pa.generate_seismic_data_from_file(fresh, solver, save_method=method)
     for a, b in zip(shots, fresh):
         assert np.array_equal(a.receivers.data, b.receivers.data)
         assert np.array_equal(a.receivers.ts, b.receivers.ts)
