@@ -372,6 +372,7 @@ def run_3d(args):
         return tot * 1e3 / max(n, 1), n
 
     b = DeviceBatch(geos, model_u, mesh.deltas, solver.dt, 8, dtype=np.float32, memory_budget=budget)
+    b32_window = bool(b.ic_window)
     res = [t.clone() for t in b.forward(keep_dw=False)]
     t32 = gradient(b, res, True)
     timer = b.timer
@@ -415,7 +416,9 @@ def run_3d(args):
                    "shots": len(shots), "nt": nt, "grid_padded": list(shape), "dwaveop_segment": K,
                    "checkpoints": -(-nt // K)},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "k_step3d<float,4,2> (adjoint + imaging)",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": ("k_step3d<float,4,4> (adjoint + imaging, FP32 window sums into FP64)"
+                                if b32_window else "k_step3d<float,4,2> (adjoint + FP64 imaging)"),
                      "bytes_per_launch": b_adj, "avg_launch_us": adj_us, "launches": n_adj},
         "forward_kernel": {"kernel": "k_step3d<float,4,1> (recompute + dWaveOp store)", "avg_launch_us": fwd_us,
                            "achieved_gbs": b_fwd / (fwd_us * 1e-6) / 1e9, "bytes_per_launch": b_fwd,

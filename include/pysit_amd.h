@@ -188,7 +188,7 @@ typedef struct {
      * same shape and window as the model); rhs_k = -m1 * dWaveOp0_k with
      * dWaveOp0 read from dw                                                 */
     const void *born_m1;
-    /* two-level imaging accumulator (FP32 2D batches, adjoint): when ic32 is
+    /* two-level imaging accumulator (FP32 batches, adjoint): when ic32 is
      * not NULL the per-step products are summed into ic32 (FP32, total_n,
      * zero on entry) and ic32 is added into ic (FP64) and cleared every
      * ic_window slots and at the end of every pysit_amd_batch_run call, so ic

@@ -294,8 +294,8 @@ static int batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int slo
              "3D state buffers missing");
     PA_CHECK(B.imaging_period >= 1, "imaging_period must be >= 1");
     PA_CHECK(mode != 2 || !B.dw || B.ic, "imaging needs the FP64 accumulator");
-    PA_CHECK(!B.ic32 || (B.dim == 2 && B.dtype == 0 && B.ic && B.ic_window >= 1),
-             "the FP32 window accumulator needs a 2D FP32 batch, ic and ic_window >= 1");
+    PA_CHECK(!B.ic32 || (B.dtype == 0 && B.ic && B.ic_window >= 1),
+             "the FP32 window accumulator needs an FP32 batch, ic and ic_window >= 1");
     PA_CHECK(mode == 0 || mode == 2 || B.dw, "modes 1 and 3 need the dWaveOp store");
     PA_CHECK(B.n_rec == 0 || (B.recs && B.rec_idx && B.rec_w), "receiver tables missing");
     PA_CHECK(seg_base >= 0 && seg_base <= slot_begin, "seg_base must not exceed slot_begin");

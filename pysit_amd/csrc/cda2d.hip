@@ -775,7 +775,7 @@ __global__ void k_ic_flush(float *__restrict__ ic32, double *__restrict__ ic, lo
     ic32[i] = 0.0f;
 }
 
-static int ic_flush(const pysit_amd_batch &B, hipStream_t st)
+int ic_flush(const pysit_amd_batch &B, hipStream_t st)
 {
     const long long n = B.total_n;
     if (n == 0) return PYSIT_AMD_OK;

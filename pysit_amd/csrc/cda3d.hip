@@ -33,6 +33,13 @@
 
 namespace pa {
 
+// Kernel modes: 0 propagate, 1 + dWaveOp store, 2 adjoint with the FP64
+// imaging accumulator, 3 Born, 4 adjoint with the FP32 window accumulator
+// (flushed into the FP64 one by the host loop, as in 2D).
+constexpr bool adj3(int mode) { return mode == 2 || mode == 4; }
+template <int MODE> struct Acc3D { typedef double type; };
+template <> struct Acc3D<4> { typedef float type; };
+
 constexpr int TY3 = 8;  // y rows per workgroup (one wavefront each)
 constexpr int TX3 = 8;  // x planes per tile
 
@@ -146,7 +153,7 @@ __device__ __forceinline__ void march3_int(const Ctx3D<T, R> &c, const Step3DPar
     const int co = c.co, cho = c.cho, hro = c.hro, ov = c.ov, mv = c.mv;
     const int ovd = c.ovd, ovi = c.ovi, ovf = c.ovf;
     T q[NQ], uz[TX], yh[TX], um[TX], cv[TX], fv[TX], dv[TX], mm[TX];
-    double icv[TX];
+    typename Acc3D<MODE>::type icv[TX];
 #define PA_Q(s_) q[s_] = buf_ld<T>(c.u, at(co, (x0 - R + (s_)) * pl), 0)
 #define PA_ROW(r_)                                                                                  \
     do {                                                                                            \
@@ -159,7 +166,8 @@ __device__ __forceinline__ void march3_int(const Ctx3D<T, R> &c, const Step3DPar
         fv[r] = c.finj[r * c.fstride];                                                              \
         dv[r] = (MODE >= 2) ? buf_ld<T>(c.dw, at(ovd, xp), c.shb) : T(0);                           \
         mm[r] = (MODE == 3) ? buf_ld<T>(c.m1, mv, mp) : T(0);                                       \
-        icv[r] = (MODE == 2) ? buf_ld<double>(c.ic, at(ovi, xp * (8 / SZ)), c.shb8) : 0.0;          \
+        if constexpr (MODE == 2) icv[r] = buf_ld<double>(c.ic, at(ovi, xp * (8 / SZ)), c.shb8);     \
+        if constexpr (MODE == 4) icv[r] = buf_ld<float>(c.ic, at(ovi, xp), c.shb);                    \
     } while (0)
 #pragma unroll
     for (int s = 0; s < 2 * R + D && s < NQ; ++s) PA_Q(s);
@@ -195,7 +203,8 @@ __device__ __forceinline__ void march3_int(const Ctx3D<T, R> &c, const Step3DPar
         const T c2 = cv[j] * cv[j];
         const T src = f + lap;
         const T up = fma(P.tc.dt2, c2 * src, T(2) * v - um[j]);
-        if (MODE == 2) buf_st(c.ic, at(ovi, xp * (8 / SZ)), c.shb8, icv[j] + (double)v * (double)dv[j]);
+        if constexpr (MODE == 2) buf_st(c.ic, at(ovi, xp * (8 / SZ)), c.shb8, icv[j] + (double)v * (double)dv[j]);
+        if constexpr (MODE == 4) buf_st(c.ic, at(ovi, xp), c.shb, icv[j] + v * dv[j]);
         buf_st(c.uo, at(ov, xp), 0, up);
         if (MODE == 1) buf_st(c.dw, at(ovd, xp), c.shb, c2 * src);
         if (c.has_fst) buf_st(c.fst, at(ovf, xp), c.shb, v);
@@ -220,7 +229,7 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
     T q[NQ], qx[NQ], qs[NQ];
     T uz[TX], yh[TX], sz[TX], sh[TX], pyc[TX], pyh[TX], pzc[TX], pzz[TX];
     T um[TX], cv[TX], fv[TX], dv[TX], mm[TX];
-    double icv[TX];
+    typename Acc3D<MODE>::type icv[TX];
 #define PA_Q(s_)                                                                                    \
     do {                                                                                            \
         const int xx = x0 - R + (s_);                                                               \
@@ -255,7 +264,8 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
         fv[r] = f_;                                                                                 \
         dv[r] = (MODE >= 2) ? buf_ld<T>(c.dw, in ? at(ovd, xp) : kOOB, c.shb) : T(0);               \
         mm[r] = (MODE == 3) ? buf_ld<T>(c.m1, mv, mp) : T(0);                                       \
-        icv[r] = (MODE == 2) ? buf_ld<double>(c.ic, in ? at(ovi, xp * (8 / SZ)) : kOOB, c.shb8) : 0.0; \
+        if constexpr (MODE == 2) icv[r] = buf_ld<double>(c.ic, in ? at(ovi, xp * (8 / SZ)) : kOOB, c.shb8); \
+        if constexpr (MODE == 4) icv[r] = buf_ld<float>(c.ic, in ? at(ovi, xp) : kOOB, c.shb);         \
     } while (0)
 #pragma unroll
     for (int s = 0; s < 2 * R + D && s < NQ; ++s) PA_Q(s);
@@ -378,7 +388,9 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
         const int oo = inx ? at(ov, xp) : kOOB;
         // Phi keeps its value (zero) where the node is not a PML node
         const int op = (c.gen || pml || edge) ? oo : kOOB;
-        if (MODE == 2) buf_st(c.ic, inx ? at(ovi, xp * (8 / SZ)) : kOOB, c.shb8, icv[j] + (double)uc * (double)dv[j]);
+        if constexpr (MODE == 2)
+            buf_st(c.ic, inx ? at(ovi, xp * (8 / SZ)) : kOOB, c.shb8, icv[j] + (double)uc * (double)dv[j]);
+        if constexpr (MODE == 4) buf_st(c.ic, inx ? at(ovi, xp) : kOOB, c.shb, icv[j] + uc * dv[j]);
         buf_st(c.uo, oo, 0, up);
         buf_st(c.pxo, op, 0, pxn);
         buf_st(c.pyo, op, 0, pyn);
@@ -446,7 +458,7 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
         c.nxl = S.nxl;
         c.nxr = S.nxr;
     }
-    const int taken = (MODE == 2) ? (S.nt - 1 - k) : k;
+    const int taken = adj3(MODE) ? (S.nt - 1 - k) : k;
     const int cur = taken & 1;
     const long long off = S.off;
     const long long nb = (long long)nx * ny * zs * SZ;
@@ -471,7 +483,7 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
     c.rhs = make_rsrc(c.has_rhs ? (const void *)P.rhs : vb, c.has_rhs ? nb : 0);
     c.shb = (int)(off * SZ);
     c.shb8 = (int)(off * 8);
-    const bool img = (MODE == 2) && (k % B.imaging_period == 0) && B.dw;
+    const bool img = adj3(MODE) && (k % B.imaging_period == 0) && B.dw;
     const bool store = (MODE == 1) && (k % B.imaging_period == 0) && B.dw;
     const long long slot_el = (long long)(slot - seg_base) * B.total_n;
     const long long slot_bytes = B.total_n * SZ;
@@ -479,11 +491,12 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
     c.dw = make_rsrc(dwon ? (const void *)((const T *)B.dw + slot_el) : vb, dwon ? slot_bytes : 0);
     c.has_fst = B.fstore != nullptr;
     c.fst = make_rsrc(c.has_fst ? (const void *)((const T *)B.fstore + slot_el) : vb, c.has_fst ? slot_bytes : 0);
-    c.ic = make_rsrc(MODE == 2 ? (const void *)B.ic : vb, MODE == 2 ? B.total_n * 8 : 0);
+    if constexpr (MODE == 4) c.ic = make_rsrc(B.ic32, B.total_n * 4);
+    else c.ic = make_rsrc(MODE == 2 ? (const void *)B.ic : vb, MODE == 2 ? B.total_n * 8 : 0);
     c.m1 = make_rsrc(MODE == 3 ? B.born_m1 : vb,
                      MODE == 3 ? (long long)(S.mx_hi + 1) * S.mny * S.mnz * SZ : 0);
     c.ovd = dwon ? c.ov : kOOB;
-    c.ovi = img ? c.ov8 : kOOB;
+    c.ovi = img ? (MODE == 4 ? c.ov : c.ov8) : kOOB;
     c.ovf = c.has_fst ? c.ov : kOOB;
     // PML profiles: lane y/z values and the tile planes' x profile (lane p:
     // kernel form of plane x0+p, lane TX+p: mesh form), masked buffer loads
@@ -578,7 +591,7 @@ void k_step3d(const pysit_amd_batch B, const Step3DParams<T> P,
     // the tile blocks, then the receiver blocks (they fill the launch's tail)
     const int b = blockIdx.x;
     const int tile_blocks = gridDim.x - rec_blocks;
-    if (MODE != 2 && b >= tile_blocks) {
+    if (!adj3(MODE) && b >= tile_blocks) {
         const int rec = (b - tile_blocks) * (TY3 * 4) + (threadIdx.x >> 4);
         sample_group3<T>(B, slot, rec < n_rec_lanes ? rec : B.n_rec, threadIdx.x & 15);
         return;
@@ -597,7 +610,7 @@ template <typename T, int R, int MODE>
 static int launch3d(const pysit_amd_batch &B, const Step3DParams<T> &P, int slot, int seg_base,
                     int n_tiles, int n_recs, hipStream_t st, const int *cursor = nullptr)
 {
-    const int recs = (MODE != 2) ? n_recs : 0;
+    const int recs = adj3(MODE) ? 0 : n_recs;
     const int rec_blocks = recs ? (recs + TY3 * 4 - 1) / (TY3 * 4) : 0;
     const int blocks = n_tiles + rec_blocks;
     if (blocks == 0) return PYSIT_AMD_OK;
@@ -621,23 +634,31 @@ static int launch3d_r(int r, const pysit_amd_batch &B, const Step3DParams<T> &P,
     return PYSIT_AMD_EINVAL;
 }
 
+int ic_flush(const pysit_amd_batch &B, hipStream_t st);
+
 template <typename T>
 struct Sink3D final : StepSink {
     const pysit_amd_batch &B;
     const Step3DParams<T> &P;
     int r, mode;
-    Sink3D(const pysit_amd_batch &B_, const Step3DParams<T> &P_, int r_, int mode_)
-        : B(B_), P(P_), r(r_), mode(mode_) {}
+    bool window;
+    Sink3D(const pysit_amd_batch &B_, const Step3DParams<T> &P_, int r_, int mode_, bool window_)
+        : B(B_), P(P_), r(r_), mode(mode_), window(window_) {}
     int step(int slot, int seg_base, int nt, int nr, const int *cursor, hipStream_t st) override
     {
         switch (mode) {
         case 0: return launch3d_r<T, 0>(r, B, P, slot, seg_base, nt, nr, st, cursor);
         case 1: return launch3d_r<T, 1>(r, B, P, slot, seg_base, nt, nr, st, cursor);
-        case 2: return launch3d_r<T, 2>(r, B, P, slot, seg_base, nt, 0, st, cursor);
+        case 2:
+            if (!window) return launch3d_r<T, 2>(r, B, P, slot, seg_base, nt, 0, st, cursor);
+            if constexpr (std::is_same<T, float>::value)
+                return launch3d_r<T, 4>(r, B, P, slot, seg_base, nt, 0, st, cursor);
+            set_error("the FP32 window accumulator needs an FP32 batch");
+            return PYSIT_AMD_EINVAL;
         default: return launch3d_r<T, 3>(r, B, P, slot, seg_base, nt, nr, st, cursor);
         }
     }
-    int flush(hipStream_t) override { return PYSIT_AMD_OK; }
+    int flush(hipStream_t st) override { return ic_flush(B, st); }
 };
 
 template <typename T>
@@ -654,8 +675,10 @@ int run3d(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int 
     P.tc = time_coef<T>(B.dt);
     P.rhs = nullptr;
     P.single = 0;
-    Sink3D<T> S(B, P, r, mode);
-    const LoopSpec L{mode, slot_begin, slot_end, direction, seg_base, active_tiles, active_recs, 0};
+    const bool window = mode == 2 && B.ic32 != nullptr;
+    Sink3D<T> S(B, P, r, mode, window);
+    const LoopSpec L{mode, slot_begin, slot_end, direction, seg_base, active_tiles, active_recs,
+                     window ? B.ic_window : 0};
     return time_loop(plan, B, L, S, st);
 }
 

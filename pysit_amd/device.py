@@ -166,12 +166,12 @@ class DeviceBatch:
         self.tile_y = N.tile_rows(self.dim, 1 if self.dtype == np.float64 else 0, 1) if self.dim == 3 else 1
         self.zalign = z_align(self.dtype.itemsize)
         self.memory_budget = memory_budget
-        # FP32 2D adjoints sum the imaging condition in FP32 over windows of
+        # FP32 adjoints sum the imaging condition in FP32 over windows of
         # `ic_window` slots and in FP64 across windows (0: FP64 every step)
         if ic_window is None:
-            ic_window = IC_WINDOW if (self.dim == 2 and self.dtype == np.float32) else 0
-        if ic_window and not (self.dim == 2 and self.dtype == np.float32):
-            raise ValueError("the FP32 window accumulator is for 2D FP32 batches")
+            ic_window = IC_WINDOW if self.dtype == np.float32 else 0
+        if ic_window and self.dtype != np.float32:
+            raise ValueError("the FP32 window accumulator is for FP32 batches")
         self.ic_window = int(ic_window)
         # tile table order: "shot" (all tiles of a shot together: halo rows of
         # neighbouring tiles are reused in L2) or "interleave" (the shots of

@@ -63,7 +63,7 @@ class TemporalModeling(object):
         self.solver = solver
         self.memory_budget = memory_budget
         self.tile_x = tile_x
-        self.ic_window = ic_window  # None: the batch default (FP32 2D: IC_WINDOW = 64 slots)
+        self.ic_window = ic_window  # None: the batch default (FP32: IC_WINDOW = 64 slots)
         # launch configuration (tile rows, tile order, graphs) picked by timing
         # the candidates once per batch signature, unless tile_x is given
         self.autotune = bool(autotune)

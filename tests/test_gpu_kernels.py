@@ -208,6 +208,25 @@ def test_ic_window_accumulator_matches_fp64_accumulation(gpu):
         assert rel_l2(img, ref) < 1e-5
 
 
+def test_ic_window_accumulator_3d_matches_fp64_accumulation(gpu):
+    # the same in 3D (k_step3d mode 4): windows of 7 and 64 slots, and
+    # checkpointed segments of 30 slots, against FP64 accumulation every step
+    p = Synthetic((24, 26, 70), ((4, 4), (3, 5), (4, 3)), (0.01, 0.011, 0.012), 130, 8, seed=8, n_rec=6)
+    rng = np.random.default_rng(9)
+    res = [rng.standard_normal((p.nt, p.n_rec))]
+    imgs = []
+    for w, seg in ((0, None), (7, None), (7, 30), (64, None)):
+        b = _batch([p], np.float32, ic_window=w)
+        assert b.ic_window == w
+        b.forward(keep_dw=True, segment=seg)
+        b.migrate(res)
+        imgs.append(b.image(0).cpu().numpy().reshape(-1).copy())
+    ref = imgs[0]
+    assert np.linalg.norm(ref) > 0
+    for img in imgs[1:]:
+        assert rel_l2(img, ref) < 1e-5
+
+
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-10), (np.float32, 1e-3)])
 def test_batch_3d(gpu, dtype, tol):
     p = Synthetic((20, 22, 70), ((4, 4), (3, 5), (4, 3)), (0.01, 0.011, 0.012), 90, 8, seed=5, n_rec=5)
