@@ -45,8 +45,14 @@ constexpr int TX3 = 8;  // x planes per tile
 
 int tile_rows_3d(int axis) { return axis == 0 ? TX3 : TY3; }
 
+// x planes whose loads are in flight ahead of the plane being computed: 1
+// in PML tiles (their march is at the 128-VGPR limit; 2 spills), 2 in plain
+// tiles (2 % faster steps than 1; 3 and 4 no faster).
 #ifndef PA3D_PREFETCH
 #define PA3D_PREFETCH 1
+#endif
+#ifndef PA3D_PREFETCH_INT
+#define PA3D_PREFETCH_INT 2
 #endif
 
 template <typename T>
@@ -147,7 +153,7 @@ template <typename T, int R, int MODE>
 __device__ __forceinline__ void march3_int(const Ctx3D<T, R> &c, const Step3DParams<T> &P, Lds3D<T, R> &L,
                                            int x0)
 {
-    constexpr int TX = TX3, D = PA3D_PREFETCH, NQ = TX + 2 * R, LZ = Lds3D<T, R>::LZ;
+    constexpr int TX = TX3, D = PA3D_PREFETCH_INT, NQ = TX + 2 * R, LZ = Lds3D<T, R>::LZ;
     constexpr int SZ = sizeof(T);
     const int pl = c.plane;
     const int co = c.co, cho = c.cho, hro = c.hro, ov = c.ov, mv = c.mv;
