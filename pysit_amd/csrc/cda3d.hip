@@ -673,6 +673,9 @@ int run3d(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int 
           hipStream_t st)
 {
     PA_CHECK(B.tile_x == TX3 && B.tile_y == TY3, "3D tiles are 8 x 8 (tile_x, tile_y)");
+    // every per-batch array (the FP64 accumulator at 8 B per node the
+    // largest) is reached through 32-bit buffer offsets
+    PA_CHECK(B.total_n * 8 <= kMaxRsrcBytes, "batch too large for one launch (total_n)");
     Step3DParams<T> P{};
     const int r = B.order / 2;
     P.cx = axis_coef<T>(r, B.dx);
@@ -716,6 +719,7 @@ int step3d_entry(int order, const T *km1_u, const T *k_phix, const T *k_phiy, co
                  kp1_phiz != k_phiz && kp1_psi != k_psi,
              "outputs must not alias inputs");
     const long long N = (long long)nx * ny * nz;
+    PA_CHECK(N * (long long)sizeof(T) <= kMaxRsrcBytes, "grid too large for one step launch");
     PA_HIP(hipMemcpyAsync(kp1_u, km1_u, N * sizeof(T), hipMemcpyDeviceToDevice, st));
     pysit_amd_batch B{};
     B.dtype = sizeof(T) == 8;

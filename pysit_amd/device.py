@@ -41,6 +41,31 @@ def z_align(itemsize):
     return 128 // itemsize
 
 
+# Largest region a launch reaches through 32-bit buffer offsets (the
+# kernels' kMaxRsrcBytes, common.h)
+MAX_REGION_BYTES = 0x7FFF0000
+
+
+def max_batch_nodes(dim, itemsize):
+    """Most padded nodes one batch may hold: the FP64 imaging accumulator
+    (8 B per node) and, in 2D, the six state buffers, kept in one region,
+    must each fit one buffer resource."""
+    return MAX_REGION_BYTES // (max(8, 6 * itemsize) if dim == 2 else 8)
+
+
+def shot_nodes(shape, itemsize):
+    """Padded nodes of one shot of grid ``shape`` as a batch stores it (rows
+    along z at a whole number of cache lines)."""
+    a = z_align(itemsize)
+    return int(np.prod(shape[:-1])) * (-(-int(shape[-1]) // a) * a)
+
+
+def shots_per_batch(shape, itemsize):
+    """Most shots of grid ``shape`` that fit one batch (at least 1; a shot
+    too large on its own is refused when its batch is built)."""
+    return max(1, max_batch_nodes(len(shape), itemsize) // shot_nodes(shape, itemsize))
+
+
 def z_tiles_2d(nz, align=1):
     """z tiles (2D and 3D): (start, owned end) per tile.  Tiles start at 0, 64,
     128, ... except the last, which starts at z_last_tile(nz, align) and
@@ -290,6 +315,10 @@ class DeviceBatch:
             off += sn
             tiles_info.append(self._tiles_of(s))
         self.total_n = off
+        limit = max_batch_nodes(self.dim, self.dtype.itemsize)
+        if off > limit:
+            raise ValueError(f"{len(self.shots)} shots of {self.dim}D grids hold {off} padded nodes; one batch "
+                             f"holds at most {limit}: split the shots (batch_size=)")
         self.tiles_info = tiles_info
 
         # tile table: interleave the shots of one tile when they are alike,

@@ -26,6 +26,7 @@ import time
 
 import numpy as np
 
+from ..device import shots_per_batch
 from .temporal_modeling import TemporalModeling
 
 __all__ = ["generate_seismic_data", "generate_shot_data_time", "generate_seismic_data_from_file",
@@ -150,7 +151,8 @@ def generate_seismic_data(shots, solver, model, verbose=False, frequencies=None,
     tm = TemporalModeling(solver)
     solver.model_parameters = model
     ts = solver.ts()
-    step = batch_size or len(shots)
+    step = batch_size or shots_per_batch(solver.mesh.shape(include_bc=True, as_grid=True),
+                                         np.dtype(solver.dtype).itemsize)
     for i in range(0, len(shots), step):
         group = shots[i:i + step]
         traces = tm.forward_model_batch(group, model)

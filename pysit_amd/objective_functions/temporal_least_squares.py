@@ -20,6 +20,7 @@ the wrap is a no-op.
 """
 import numpy as np
 
+from ..device import shots_per_batch
 from ..modeling.temporal_modeling import TemporalModeling
 
 __all__ = ["TemporalLeastSquares", "ObservedOnDevice"]
@@ -97,7 +98,9 @@ class TemporalLeastSquares(object):
         return self.parallel_wrap_shot.use_parallel
 
     def _groups(self, shots):
-        step = self.batch_size or len(shots)
+        # by default as many shots per batch as one launch can address
+        step = self.batch_size or shots_per_batch(self.solver.mesh.shape(include_bc=True, as_grid=True),
+                                                  np.dtype(self.solver.dtype).itemsize)
         for i in range(0, len(shots), step):
             yield shots[i:i + step]
 

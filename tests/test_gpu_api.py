@@ -160,3 +160,35 @@ def test_lbfgs_two_iterations_decrease_misfit(gpu):
     res = opt(shots, m0, 2, status_configuration={"objective_frequency": 1})
     f2 = obj.evaluate(shots, res)
     assert f2 < f0
+
+
+def test_default_batches_split_at_the_addressable_region(gpu, monkeypatch):
+    """With a region that holds one shot, the objective and the data
+    generation run one batch per shot by default and give what one batch of
+    all shots gives; an explicit batch beyond the region is refused."""
+    import pysit_amd as pa
+    from pysit_amd import device
+    m, _, solver, model = c1_setup(order=4, tf=0.3)
+    shots = pa.equispaced_acquisition(m, pa.RickerWavelet(10.0), sources=3, source_depth=0.05,
+                                      receivers="max", receiver_depth=0.05)
+    pa.generate_seismic_data(shots, solver, model)
+    C0 = model.C.copy()
+    C0[:] = 1.0
+    m0 = solver.ModelParameters(m, {"C": C0})
+    obj = pa.TemporalLeastSquares(solver)
+    g1, f1 = obj.compute_gradient(shots, m0), obj.evaluate(shots, m0)
+
+    shape = tuple(solver.mesh.shape(include_bc=True, as_grid=True))
+    n = device.shot_nodes(shape, 8)
+    monkeypatch.setattr(device, "MAX_REGION_BYTES", 48 * n)  # six FP64 state buffers of one shot
+    assert device.shots_per_batch(shape, 8) == 1
+    split = copy.deepcopy(shots)
+    pa.generate_seismic_data(split, solver, model)
+    for a, b in zip(shots, split):
+        assert np.array_equal(a.receivers.data, b.receivers.data)
+    obj = pa.TemporalLeastSquares(solver)
+    g2, f2 = obj.compute_gradient(shots, m0), obj.evaluate(shots, m0)
+    assert rel_l2(g2.data.reshape(-1), g1.data.reshape(-1)) < 1e-13
+    assert abs(f2 - f1) <= 1e-13 * abs(f1)
+    with pytest.raises(ValueError, match="one batch holds at most"):
+        pa.TemporalModeling(solver).forward_model_batch(shots, model)

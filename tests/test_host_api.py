@@ -323,3 +323,24 @@ def test_shot_data_file_errors(tmp_path, monkeypatch):
         pickle.dump(math.sqrt, fh)  # a global other than numpy's array rebuild
     with pytest.raises(IOError, match="different save_method"):
         pa.load_shot_data(shots, "pickle")
+
+
+def test_batch_limits_follow_the_kernels_addressable_region():
+    """A batch's per-node arrays are reached through 32-bit buffer offsets:
+    the host's limit is the kernels' kMaxRsrcBytes, and the default batch
+    split keeps every batch within it (C4's eight 288^3 shots, C2's 16
+    shots: one batch each)."""
+    from pysit_amd import device
+    src = open(os.path.join(ROOT, "pysit_amd", "csrc", "common.h")).read()
+    assert int(re.search(r"kMaxRsrcBytes = (0x[0-9a-fA-F]+)LL", src).group(1), 16) == device.MAX_REGION_BYTES
+    lim = device.MAX_REGION_BYTES
+    n3 = device.shot_nodes((288, 288, 288), 4)
+    assert n3 == 288 ** 3
+    k3 = device.shots_per_batch((288, 288, 288), 4)
+    assert k3 == 11 and k3 * n3 * 8 <= lim < (k3 + 1) * n3 * 8   # FP64 accumulator
+    n2 = device.shot_nodes((1041, 341), 4)
+    assert n2 == 1041 * 352                                         # rows at 128 B
+    k2 = device.shots_per_batch((1041, 341), 4)
+    assert k2 * n2 * 24 <= lim < (k2 + 1) * n2 * 24                 # six FP32 state buffers
+    assert device.shots_per_batch((1041, 341), 8) == k2 // 2
+    assert device.shots_per_batch((2000, 2000, 2000), 4) == 1
