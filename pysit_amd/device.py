@@ -680,7 +680,9 @@ class DeviceBatch:
         return self.traces_in_caller_order()
 
     def _field_store(self):
-        return torch.empty((self.max_nt, self.total_n), dtype=self.tdtype, device=self.dev)
+        # zeroed: no step writes the slots before an uneven shot starts or
+        # the row padding past nz, and the store is returned whole
+        return torch.zeros((self.max_nt, self.total_n), dtype=self.tdtype, device=self.dev)
 
     def load_dw(self, per_shot_images):
         """Use host-provided dWaveOp (per shot, caller order: a sequence of

@@ -245,12 +245,12 @@ def test_batch_3d(gpu, dtype, tol):
 
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-10), (np.float32, 1e-3)])
 def test_batch_3d_with_plain_tiles(gpu, dtype, tol):
-    # 32 x 36 x 140 nodes with 4-node PML: the x-y tiles in the middle and the
-    # z tile at 64..127 are plain tiles (no PML node within r of them); the
-    # source and the receivers sit in them, and the whole last wavefield is
-    # compared too
-    p = Synthetic((32, 36, 140), ((4, 4), (4, 4), (4, 4)), (0.01, 0.011, 0.012), 60, 8, seed=7, n_rec=9,
-                  src_center=(16, 18, 92), rec_z=96)
+    # 64 x 36 x 140 nodes with 4-node PML: the x-y tiles in the middle (x
+    # tiles of 16 planes, y tiles of 8 rows) and the z tile at 64..127 are
+    # plain tiles (no PML node within r of them); the source and the
+    # receivers sit in them, and the whole last wavefield is compared too
+    p = Synthetic((64, 36, 140), ((4, 4), (4, 4), (4, 4)), (0.01, 0.011, 0.012), 60, 8, seed=7, n_rec=9,
+                  src_center=(32, 18, 92), rec_z=96)
     b = _batch([p], dtype)
     flags = b.tiles.view(-1, 4)[:, 2].cpu().numpy()
     assert ((flags & 1) == 0).sum() >= 4  # bit0: the PML path
