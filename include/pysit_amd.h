@@ -171,8 +171,14 @@ typedef struct {
     const long long *rec_idx;
     const double *rec_w;
     /* state: each field has two buffers; the local step parity picks which
-     * holds time level k.  2D uses u, phix, phiz; 3D adds phiy and psi.     */
+     * holds time level k.  2D uses u, phix, phiz; 3D adds phiy and psi.
+     * du (one buffer, required) holds the increment du_k = u_k - u_{k-1},
+     * updated in place: a step computes du_{k+1} = du_k + (update) and
+     * u_{k+1} = u_k + du_{k+1} (the reference's leapfrog in increment form;
+     * in FP32 it keeps the low bits of the time difference that
+     * 2u_k - u_{k-1} loses).  A batch at rest has u = du = 0.               */
     void *u[2], *phix[2], *phiy[2], *phiz[2], *psi[2];
+    void *du;
     long long total_n;     /* sum of shot grid sizes                             */
     /* dWaveOp store (dtype of the batch): slot-major, dw[(slot - seg_base) *
      * total_n + off + idx]; NULL = not stored                               */
@@ -206,6 +212,18 @@ typedef struct {
     const int *inj_node_shot;
     int n_inj_nodes;
     int inj_pre_slots;
+    /* dWaveOp form (what mode 1 stores and mode 3 reads):
+     *   0  PySIT's (constant_density_acoustic_base.py:24-40): the step's time
+     *      operator with the MESH PML profile, whose right side sits one node
+     *      off the kernel's (the reference's index quirk);
+     *   1  exact (2D only): the step's time operator with the KERNEL profile,
+     *      i.e. C^2 (rhs + L u + Dx Phix + Dz Phiz) by the step identity --
+     *      the derivative of the discrete step with respect to m = C^-2, so
+     *      the imaging condition against the adjoint recursion is the exact
+     *      gradient of the discrete misfit (that recursion is the step's
+     *      transpose after the change of variables Phi' = -dt (sz - sx) mu).
+     * Both forms agree wherever the two profiles do (off the right PML).   */
+    int dw_form;
 } pysit_amd_batch;
 
 /* Run slots [slot_begin, slot_end) forward (direction = +1) or

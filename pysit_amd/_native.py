@@ -43,12 +43,14 @@ class Batch(ctypes.Structure):
         ("inj_ptr", _vp), ("inj_node", _vp), ("inj_nptr", _vp), ("inj_chan", _vp), ("inj_w", _vp),
         ("n_rec", _i), ("recs", _vp), ("rec_idx", _vp), ("rec_w", _vp),
         ("u", _vp * 2), ("phix", _vp * 2), ("phiy", _vp * 2), ("phiz", _vp * 2), ("psi", _vp * 2),
+        ("du", _vp),
         ("total_n", _ll),
         ("dw", _vp), ("ic", _vp),
         ("imaging_period", _i),
         ("fstore", _vp), ("born_m1", _vp),
         ("ic32", _vp), ("ic_window", _i),
         ("inj_pre", _vp), ("inj_node_shot", _vp), ("n_inj_nodes", _i), ("inj_pre_slots", _i),
+        ("dw_form", _i),
     ]
 
 

@@ -20,6 +20,9 @@ class ReceiverBase(object):
     def __init__(self, time_window=None, **kwargs):
         self.ts = None
         self._data = None
+        # bumped whenever the data change through this class (assignment or
+        # sampling into them), so device copies know when to upload again
+        self.data_version = 0
         self.shot = None
         self._time_window = ("None",) if time_window is None else time_window
 
@@ -33,6 +36,7 @@ class ReceiverBase(object):
     @data.setter
     def data(self, value):
         self._data = value
+        self.data_version = getattr(self, "data_version", 0) + 1
 
     def reset_time_series(self, ts):
         self.ts = ts
@@ -81,6 +85,8 @@ class PointReceiver(PointRepresentationBase, ReceiverBase):
         v = np.asarray(v).reshape(-1)[0]
         if k is None:
             return v
+        if data is None:
+            self.data_version = getattr(self, "data_version", 0) + 1
         (self.data if data is None else data)[k] = v
 
     def extend_data_to_array(self, k=None, resid=False, data=None):
@@ -130,6 +136,7 @@ class ReceiverSet(MeshRepresentationBase, ReceiverBase):
     @data.setter
     def data(self, value):
         self._data = value
+        self.data_version = getattr(self, "data_version", 0) + 1
         c = 0
         if value is not None:
             for r in self.receiver_list:
@@ -142,6 +149,8 @@ class ReceiverSet(MeshRepresentationBase, ReceiverBase):
         v = np.asarray(v).reshape(1, -1)
         if k is None:
             return v
+        if data is None:
+            self.data_version = getattr(self, "data_version", 0) + 1
         (self.data if data is None else data)[k, :] = v
 
     def extend_data_to_array(self, k=None, resid=False, data=None):

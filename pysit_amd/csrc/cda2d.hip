@@ -56,9 +56,11 @@ namespace pa {
 // fewer halo rows.
 constexpr int kTileRows2D[2] = {8, 4};
 
-// state buffers of one step: u_k, u_{k-1} (overwritten by u_{k+1}),
-// Phix_k, Phix_{k+1}, Phiz_k, Phiz_{k+1}
-enum { F_U = 0, F_UO, F_PX, F_PXO, F_PZ, F_PZO, NF2 };
+// state buffers of one step: u_k, the other u buffer (batch: u_{k+1} is
+// written there; reference-contract step: it holds u_{k-1} and u_{k+1}
+// overwrites it), Phix_k, Phix_{k+1}, Phiz_k, Phiz_{k+1}, and (batch only)
+// the increment du_k = u_k - u_{k-1}, updated in place
+enum { F_U = 0, F_UO, F_PX, F_PXO, F_PZ, F_PZO, F_DU, NF2 };
 
 template <bool PACKED> struct State2D;
 // batch: one resource over the state region plus a byte offset per buffer
@@ -85,6 +87,7 @@ struct Step2DParams {
     const void *state;     // batch: base of the state region
     int state_bytes;       //   its extent
     int fo[3][2];          //   byte offsets of u, Phix, Phiz buffers 0 and 1
+    int fdu;               //   byte offset of the du buffer
     pysit_amd_shot shot0;  // used when single
 };
 
@@ -94,6 +97,7 @@ struct Ctx2D {
     State2D<PACKED> s;
     Rsrc model, m1, dw, fst, ic, rhs;  // dw: read (adjoint, Born) or written (forward)
     bool has_fst, has_dwo, has_rhs, has_inj, img, general_phi;
+    bool dw_exact;  // store the kernel-profile dWaveOp (C^2 src) in PML tiles too
     const T *finj;  // this lane's LDS source column (or a zero slot)
     int nxl, nxr;
     int nx, nz, lane;
@@ -269,6 +273,28 @@ __device__ __forceinline__ void dw_st(Rsrc r, int voff, int soff, T v)
     else buf_st(r, voff, soff, v);
 }
 
+// Laplacian of a node from its x row queue (ux[R] is the node, ux[R +- i]
+// its x neighbours) and its z neighbours, with difference-form taps:
+// sum_i w_i ((u_{+i} - u) + (u_{-i} - u)) per axis, the centre weight
+// -2 sum_i w_i being implied.  Neighbour differences of the smooth fields a
+// step sees round far less than the terms of the plain form
+// sum_i w_i (u_{+i} + u_{-i}) + w_0 u, which cancel to a value about (kh)^2
+// times smaller than u: over thousands of FP32 steps the difference form
+// (with the increment form of the update) keeps the traces near the FP64
+// ones.
+template <typename T, int R>
+__device__ __forceinline__ T laplacian(const Step2DParams<T> &P, const T *ux, T v, const T (&zp)[R + 1],
+                                       const T (&zm)[R + 1])
+{
+    T lx = T(0), lz = T(0);
+#pragma unroll
+    for (int i = R; i >= 1; --i) {
+        lx = fma(P.cx.d2[i], (ux[R + i] - v) + (ux[R - i] - v), lx);
+        lz = fma(P.cz.d2[i], (zp[i] - v) + (zm[i] - v), lz);
+    }
+    return lx + lz;
+}
+
 // Plain tiles: no PML node within r of an owned node, no edge node, every
 // row of the march inside the grid.
 template <typename T, int TX, int R, int MODE, bool PACKED>
@@ -277,7 +303,8 @@ __device__ __forceinline__ void march_int(const Ctx2D<T, R, PACKED> &c, const St
     constexpr int D = is_adjoint(MODE) ? PA2D_PREFETCH_ADJ : PA2D_PREFETCH;
     constexpr int NU = TX + 2 * R;  // u rows x0-R .. x0+TX+R-1
     const int rb = c.rowb;
-    const int so_u = c.s.so(F_U), so_uo = c.s.so(F_UO);
+    const int so_u = c.s.so(F_U), so_uo = c.s.so(F_UO), so_du = c.s.so(F_DU);
+    // um: du_k (batch, increment form) or u_{k-1} (reference-contract step)
     T u[NU], uh[TX], um[TX], cv[TX], fv[TX], dv[TX], mv[TX];
     typename Acc2D<T, MODE>::type icv[TX];
     const int zv = c.zv, hv = c.hv, ov = c.ov, ovd = c.ovd, ovi = c.ovi, ovf = c.ovf, mvo = c.mv;
@@ -286,7 +313,8 @@ __device__ __forceinline__ void march_int(const Ctx2D<T, R, PACKED> &c, const St
     do {                                                                                            \
         const int r = (r_), x = x0 + r, xr = x * rb;                                                \
         uh[r] = buf_ld<T>(c.s.rs(F_U), hv + xr, so_u);                                              \
-        um[r] = buf_ld<T>(c.s.rs(F_UO), ov + xr, so_uo);                                            \
+        um[r] = PACKED ? buf_ld<T>(c.s.rs(F_DU), ov + xr, so_du)                                    \
+                       : buf_ld<T>(c.s.rs(F_UO), ov + xr, so_uo);                                   \
         const int mr = clampi(x - c.mx_off, c.mx_lo, c.mx_hi) * c.mrowb;                           \
         cv[r] = buf_ld<T>(c.model, mvo, mr);                                                        \
         fv[r] = c.finj[r * c.fstride];                                                              \
@@ -311,21 +339,20 @@ __device__ __forceinline__ void march_int(const Ctx2D<T, R, PACKED> &c, const St
         if (MODE == 3) f += mv[j] * (-dv[j]);  // Born source: m1_pad * (-dWaveOp0_k)
         T zp[R + 1], zm[R + 1];
         z_taps<T, R, PACKED>(c, 2 * (j & 1), v, uh[j], zp, zm);
-        T lx = T(0), lz = T(0);
-#pragma unroll
-        for (int i = R; i >= 1; --i) {
-            lx = fma(P.cx.d2[i], u[R + j + i] + u[R + j - i], lx);
-            lz = fma(P.cz.d2[i], zp[i] + zm[i], lz);
-        }
-        lx = fma(P.cx.d2[0], v, lx);
-        lz = fma(P.cz.d2[0], v, lz);
-        const T lap = lx + lz;
+        const T lap = laplacian<T, R>(P, u + j, v, zp, zm);
         const T c2 = cv[j] * cv[j];
         const T src = f + lap;
-        const T up = fma(P.tc.dt2, c2 * src, T(2) * v - um[j]);
+        const T S = c2 * src;
         if (is_adjoint(MODE)) ic_store<T, MODE>(c, ovi, xr, icv[j], v, dv[j]);
-        buf_st(c.s.rs(F_UO), ov + xr, so_uo, up);
-        if (MODE == 1) dw_st<T>(c.dw, ovd + xr, c.shb, c2 * src);
+        if constexpr (PACKED) {
+            // increment form: du_{k+1} = du_k + dt^2 S, u_{k+1} = u_k + du_{k+1}
+            const T dn = fma(P.tc.dt2, S, um[j]);
+            buf_st(c.s.rs(F_UO), ov + xr, so_uo, v + dn);
+            buf_st(c.s.rs(F_DU), ov + xr, so_du, dn);
+        } else {
+            buf_st(c.s.rs(F_UO), ov + xr, so_uo, fma(P.tc.dt2, S, T(2) * v - um[j]));
+        }
+        if (MODE == 1) dw_st<T>(c.dw, ovd + xr, c.shb, S);
         if (c.has_fst) buf_st(c.fst, ovf + xr, c.shb, v);
     }
 #undef PA_LD_U
@@ -343,6 +370,7 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
     constexpr int NU = TX + 2 * R;
     const int rb = c.rowb, nx = c.nx;
     const int so_u = c.s.so(F_U), so_uo = c.s.so(F_UO), so_px = c.s.so(F_PX), so_pz = c.s.so(F_PZ);
+    const int so_du = c.s.so(F_DU);
     // values, not fields: a conditional on two fields would select their
     // addresses and keep the context in memory
     const int zv = c.zv, zpv = c.zpv, hv = c.hv, hpv = c.hpv, ov = c.ov, ovd = c.ovd, ovi = c.ovi, ovf = c.ovf;
@@ -367,7 +395,7 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
         const bool xp = !XC && (x < c.nxl || x >= nx - c.nxr);                                      \
         const int ovr = own ? ov + xr : kOOB;                                                       \
         uh[r] = buf_ld<T>(c.s.rs(F_U), own ? hv + xr : kOOB, so_u);                                 \
-        um[r] = buf_ld<T>(c.s.rs(F_UO), ovr, so_uo);                                                \
+        um[r] = PACKED ? buf_ld<T>(c.s.rs(F_DU), ovr, so_du) : buf_ld<T>(c.s.rs(F_UO), ovr, so_uo); \
         const int mr = clampi(xs - c.mx_off, c.mx_lo, c.mx_hi) * c.mrowb;                          \
         cv[r] = buf_ld<T>(c.model, mvo, mr);                                                        \
         T f = PACKED ? T(0) : buf_ld<T>(c.rhs, ovr, 0); /* dense rhs: reference-contract step */   \
@@ -390,6 +418,7 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
     const T dtszk = P.tc.dt * szk;
     const T fac1z = P.tc.two_dt2 / (T(2) + dtszk);
     const T dsz = szm - szk;
+    const T szk_dt = szk * P.tc.inv_dt;
 #pragma unroll
     for (int j = 0; j < TX; ++j) {
         if (j + 2 * R + D < NU) PA_LD_U(j + 2 * R + D);
@@ -406,17 +435,9 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
         T zp[R + 1], zm[R + 1], pzp[R + 1], pzm[R + 1];
         z_taps<T, R, PACKED>(c, 2 * (j & 1), v, uh[j], zp, zm);
         z_taps<T, R, PACKED>(c, 2 * (j & 1) + 1, pv[j], ph[j], pzp, pzm);
-        T lx = T(0), lz = T(0);
-#pragma unroll
-        for (int i = R; i >= 1; --i) {
-            lx = fma(P.cx.d2[i], u[R + j + i] + u[R + j - i], lx);
-            lz = fma(P.cz.d2[i], zp[i] + zm[i], lz);
-        }
-        lx = fma(P.cx.d2[0], v, lx);
-        lz = fma(P.cz.d2[0], v, lz);
-        const T lap = lx + lz;
+        const T lap = laplacian<T, R>(P, u + j, v, zp, zm);
         const T c2 = cv[j] * cv[j];
-        const T u_m = um[j];
+        const T dm = um[j];
         const bool edge = XC ? c.zedge : ((x == 0) | (x == nx - 1) | c.zedge);
 
         T dux = T(0), duz = T(0), dpx = T(0), dpz = T(0);
@@ -428,12 +449,15 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
             dpz = fma(P.cz.d1[i - 1], pzp[i] - pzm[i], dpz);
         }
         const T src = ((f + lap) + dpx) + dpz;
+        const T S = c2 * src;
         const T sx = XC ? T(0) : lane_bcast(vsig, j);
         const T sxmj = XC ? T(0) : lane_bcast(vsig, TX + j);
         const T dt = P.tc.dt;
         const T pxc = px[R + j], pzc = pv[j];
-        const T upi = fma(P.tc.dt2, c2 * src, T(2) * v - u_m);
-        T pxn, pzn, up, dw;
+        // undamped update: increment form (batch) or the reference's form
+        const T dni = fma(P.tc.dt2, S, dm);
+        const T upi = PACKED ? v + dni : fma(P.tc.dt2, S, T(2) * v - dm);
+        T pxn, pzn, up, dn = T(0), dw;
         bool pml;
         if (XC || (sx == T(0) && sxmj == T(0) && x > 0 && x < nx - 1)) {
             // a row outside the x PML (wave-uniform): the damped update below
@@ -442,29 +466,46 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
             pml = szk != T(0);
             const T pxd = pxc + dtszk * dux;
             const T pzd = (pzc - dtszk * pzc) + (-dtszk) * duz;
-            const T fac2 = (c2 * src - (u_m - T(2) * v) * P.tc.inv_dt2) + szk * u_m * P.tc.inv_2dt;
             pxn = pml ? pxd : pxc;
             pzn = pml ? pzd : pzc;
-            up = pml ? fac1z * fac2 : upi;
-            dw = c2 * src + dsz * (up - u_m) * P.tc.inv_2dt;
+            if constexpr (PACKED) {
+                dn = pml ? dm + fac1z * (S - szk_dt * dm) : dni;
+                up = v + dn;
+                dw = S + dsz * (dn + dm) * P.tc.inv_2dt;
+            } else {
+                const T fac2 = (S - (dm - T(2) * v) * P.tc.inv_dt2) + szk * dm * P.tc.inv_2dt;
+                up = pml ? fac1z * fac2 : upi;
+                dw = S + dsz * (up - dm) * P.tc.inv_2dt;
+            }
         } else {
             pml = sx != T(0) || szk != T(0);
             // damped update (PML node) and plain update, selected per lane
             const T pxd = (pxc - dt * sx * pxc) + dt * (szk - sx) * dux;
             const T pzd = (pzc - dt * szk * pzc) + dt * (sx - szk) * duz;
             const T ss = sx + szk;
+            const T pp = sx * szk;
             const T fac1 = P.tc.two_dt2 / (T(2) + dt * ss);
-            const T fac2 = c2 * src - (u_m - T(2) * v) * P.tc.inv_dt2 + ss * u_m * P.tc.inv_2dt - (sx * szk) * v;
             pxn = pml ? pxd : pxc;
             pzn = pml ? pzd : pzc;
-            up = pml ? fac1 * fac2 : upi;
             // dWaveOp with the mesh profile: the step identity gives
-            // (u+ - 2u + u-)/dt^2 = c^2 src - sum(sk)(u+ - u-)/(2dt) - prod(sk) u
-            dw = c2 * src + ((sxmj + szm) - (sx + szk)) * (up - u_m) * P.tc.inv_2dt + (sxmj * szm - sx * szk) * v;
+            // (u+ - 2u + u-)/dt^2 = S - sum(sk)(u+ - u-)/(2dt) - prod(sk) u
+            if constexpr (PACKED) {
+                // du_{k+1} = du_k + fac1 (S - prod(sk) u - sum(sk) du_k / dt):
+                // the reference's damped update with u_{k-1} = u_k - du_k
+                dn = pml ? dm + fac1 * ((S - pp * v) - ss * dm * P.tc.inv_dt) : dni;
+                up = v + dn;
+                dw = S + ((sxmj + szm) - ss) * (dn + dm) * P.tc.inv_2dt + (sxmj * szm - pp) * v;
+            } else {
+                const T fac2 = S - (dm - T(2) * v) * P.tc.inv_dt2 + ss * dm * P.tc.inv_2dt - pp * v;
+                up = pml ? fac1 * fac2 : upi;
+                dw = S + ((sxmj + szm) - ss) * (up - dm) * P.tc.inv_2dt + (sxmj * szm - pp) * v;
+            }
         }
+        if (c.dw_exact) dw = S;
         bool phi_store = pml || c.general_phi;
         if (edge) {
             up = T(0);
+            dn = T(0);
             pxn = T(0);
             pzn = T(0);
             dw = T(0);
@@ -472,6 +513,7 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
         }
         if (is_adjoint(MODE)) ic_store<T, MODE>(c, own ? ovi : kOOB, xr, icv[j], v, dv[j]);
         buf_st(c.s.rs(F_UO), ovr, so_uo, up);
+        if constexpr (PACKED) buf_st(c.s.rs(F_DU), ovr, so_du, dn);
         const int op = phi_store ? ovr : kOOB;
         buf_st(c.s.rs(F_PXO), op, c.s.so(F_PXO), pxn);
         buf_st(c.s.rs(F_PZO), op, c.s.so(F_PZO), pzn);
@@ -535,6 +577,7 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
         c.s.o[F_PXO] = P.fo[1][cur ^ 1] + c.shb;
         c.s.o[F_PZ] = P.fo[2][cur] + c.shb;
         c.s.o[F_PZO] = P.fo[2][cur ^ 1] + c.shb;
+        c.s.o[F_DU] = P.fdu + c.shb;
     } else {
         const long long nb = (long long)nx * S.zs * SZ;
         c.s.r[F_U] = make_rsrc(B.u[cur], nb);
@@ -553,6 +596,7 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
     c.nxl = S.nxl;
     c.nxr = S.nxr;
     c.general_phi = P.general_phi;
+    c.dw_exact = B.dw_form == 1;
     c.has_rhs = P.rhs != nullptr;
     c.rhs = make_rsrc(c.has_rhs ? (const void *)P.rhs : vbase, c.has_rhs ? (long long)nx * nz * SZ : 0);
     c.zbuf = zbuf;
@@ -825,23 +869,25 @@ int run2d(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int 
     P.rhs = nullptr;
     P.single = 0;
     P.general_phi = 0;
-    // the six state buffers share one resource: they must lie in one region
+    // the seven state buffers share one resource: they must lie in one region
     {
-        const char *f[6] = {(const char *)B.u[0], (const char *)B.u[1], (const char *)B.phix[0],
-                            (const char *)B.phix[1], (const char *)B.phiz[0], (const char *)B.phiz[1]};
+        const char *f[7] = {(const char *)B.u[0], (const char *)B.u[1], (const char *)B.phix[0],
+                            (const char *)B.phix[1], (const char *)B.phiz[0], (const char *)B.phiz[1],
+                            (const char *)B.du};
         const char *lo = f[0], *hi = f[0];
-        for (int i = 0; i < 6; ++i) {
-            PA_CHECK(f[i] != nullptr, "2D batch state buffers (u, phix, phiz) must all be given");
+        for (int i = 0; i < 7; ++i) {
+            PA_CHECK(f[i] != nullptr, "2D batch state buffers (u, phix, phiz, du) must all be given");
             lo = f[i] < lo ? f[i] : lo;
             hi = f[i] > hi ? f[i] : hi;
         }
         const long long ext = (long long)(hi - lo) + B.total_n * (long long)sizeof(T);
         PA_CHECK(ext <= kMaxRsrcBytes, "2D batch state buffers must lie in one region of at most " +
                                            std::to_string(kMaxRsrcBytes) + " bytes");
-        for (int i = 0; i < 6; ++i) {
+        for (int i = 0; i < 7; ++i) {
             PA_CHECK((f[i] - lo) % (long long)sizeof(T) == 0, "misaligned 2D batch state buffer");
-            P.fo[i / 2][i % 2] = (int)(f[i] - lo);
+            if (i < 6) P.fo[i / 2][i % 2] = (int)(f[i] - lo);
         }
+        P.fdu = (int)(f[6] - lo);
         P.state = lo;
         P.state_bytes = (int)ext;
     }

@@ -95,7 +95,9 @@ __device__ __forceinline__ void lds_barrier()
 
 template <typename T, int R>
 struct Ctx3D {
-    Rsrc u, uo, px, pxo, py, pyo, pz, pzo, ps, pso, model, m1, dw, fst, ic, rhs;
+    // u: u_k; uo: u_{k+1} (increment form) or u_{k-1}, overwritten by u_{k+1}
+    // (the reference-contract step); du: the increment du_k = u_k - u_{k-1}
+    Rsrc u, uo, du, px, pxo, py, pyo, pz, pzo, ps, pso, model, m1, dw, fst, ic, rhs;
     bool has_fst, has_rhs;
     int nx, ny, nz, lane, w, y;
     int plane;               // bytes per x plane of the shot grid
@@ -155,8 +157,26 @@ __device__ __forceinline__ double bcast3(double v, int l)
 // byte offset `o` plus plane offset `p`, kept out of range when o is
 __device__ __forceinline__ int at(int o, int p) { return o == kOOB ? kOOB : o + p; }
 
+// Laplacian with difference-form taps, sum_i w_i ((u_{+i} - u) + (u_{-i} - u))
+// per axis (the centre weight implied), from the x queue (qx[R] is the node)
+// and the LDS plane (own: the node; rows LZ apart): neighbour differences of
+// smooth fields round far less than the plain form's terms, which cancel
+// (see cda2d.hip laplacian)
+template <typename T, int R, int LZ>
+__device__ __forceinline__ T laplacian3(const Step3DParams<T> &P, const T *qx, const T *lu, int own, T v)
+{
+    T lx = T(0), lyy = T(0), lz = T(0);
+#pragma unroll
+    for (int i = R; i >= 1; --i) {
+        lx = fma(P.cx.d2[i], (qx[R + i] - v) + (qx[R - i] - v), lx);
+        lyy = fma(P.cy.d2[i], (lu[own + i * LZ] - v) + (lu[own - i * LZ] - v), lyy);
+        lz = fma(P.cz.d2[i], (lu[own + i] - v) + (lu[own - i] - v), lz);
+    }
+    return (lx + lyy) + lz;
+}
+
 // ---- plain tiles -------------------------------------------------------------
-template <typename T, int R, int MODE>
+template <typename T, int R, int MODE, bool INC>
 __device__ __forceinline__ void march3_int(const Ctx3D<T, R> &c, const Step3DParams<T> &P, Lds3D<T, R> &L,
                                            int x0)
 {
@@ -173,7 +193,7 @@ __device__ __forceinline__ void march3_int(const Ctx3D<T, R> &c, const Step3DPar
         const int r = (r_), xp = (x0 + r) * pl;                                                     \
         uz[r] = buf_ld<T>(c.u, at(cho, xp), 0);                                                     \
         yh[r] = buf_ld<T>(c.u, at(hro, xp), 0);                                                     \
-        um[r] = buf_ld<T>(c.uo, at(ov, xp), 0);                                                     \
+        um[r] = buf_ld<T>(INC ? c.du : c.uo, at(ov, xp), 0);                                        \
         const int mp = clampi(x0 + r - c.mx_off, c.mx_lo, c.mx_hi) * c.mplane;                      \
         cv[r] = buf_ld<T>(c.model, mv, mp);                                                         \
         fv[r] = c.finj[r * c.fstride];                                                              \
@@ -202,24 +222,21 @@ __device__ __forceinline__ void march3_int(const Ctx3D<T, R> &c, const Step3DPar
         const int x = x0 + j, xp = x * pl;
         T f = fv[j];
         if (MODE == 3) f += mm[j] * (-dv[j]);
-        T lx = T(0), lyy = T(0), lz = T(0);
-#pragma unroll
-        for (int i = R; i >= 1; --i) {
-            lx = fma(P.cx.d2[i], q[R + j + i] + q[R + j - i], lx);
-            lyy = fma(P.cy.d2[i], lu[own + i * LZ] + lu[own - i * LZ], lyy);
-            lz = fma(P.cz.d2[i], lu[own + i] + lu[own - i], lz);
-        }
-        lx = fma(P.cx.d2[0], v, lx);
-        lyy = fma(P.cy.d2[0], v, lyy);
-        lz = fma(P.cz.d2[0], v, lz);
-        const T lap = (lx + lyy) + lz;
+        const T lap = laplacian3<T, R, LZ>(P, q + j, lu, own, v);
         const T c2 = cv[j] * cv[j];
         const T src = f + lap;
-        const T up = fma(P.tc.dt2, c2 * src, T(2) * v - um[j]);
+        const T S = c2 * src;
         if constexpr (MODE == 2) buf_st(c.ic, at(ovi, xp * (8 / SZ)), c.shb8, icv[j] + (double)v * (double)dv[j]);
         if constexpr (MODE == 4) buf_st(c.ic, at(ovi, xp), c.shb, icv[j] + v * dv[j]);
-        buf_st(c.uo, at(ov, xp), 0, up);
-        if (MODE == 1) buf_st(c.dw, at(ovd, xp), c.shb, c2 * src);
+        if constexpr (INC) {
+            // increment form: du_{k+1} = du_k + dt^2 S, u_{k+1} = u_k + du_{k+1}
+            const T dn = fma(P.tc.dt2, S, um[j]);
+            buf_st(c.uo, at(ov, xp), 0, v + dn);
+            buf_st(c.du, at(ov, xp), 0, dn);
+        } else {
+            buf_st(c.uo, at(ov, xp), 0, fma(P.tc.dt2, S, T(2) * v - um[j]));
+        }
+        if (MODE == 1) buf_st(c.dw, at(ovd, xp), c.shb, S);
         if (c.has_fst) buf_st(c.fst, at(ovf, xp), c.shb, v);
     }
 #undef PA_Q
@@ -230,7 +247,7 @@ __device__ __forceinline__ void march3_int(const Ctx3D<T, R> &c, const Step3DPar
 // XC: no plane of the tile or within R planes of it is an x PML plane or an
 // x edge (the host flags such tiles): sigma_x is zero on every plane, the
 // terms it multiplies drop out and the damping factor is one per lane.
-template <typename T, int R, int MODE, bool XC>
+template <typename T, int R, int MODE, bool XC, bool INC>
 __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DParams<T> &P, Lds3D<T, R> &L,
                                             int x0, int hl_pz)
 {
@@ -269,7 +286,7 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
         pyh[r] = buf_ld<T>(c.py, (xb || c.bph) ? oh : kOOB, 0);                                     \
         pzc[r] = buf_ld<T>(c.pz, (xb || c.bp) ? oc : kOOB, 0);                                      \
         pzz[r] = buf_ld<T>(c.pz, (xb || c.bpz) ? oz : kOOB, 0);                                     \
-        um[r] = buf_ld<T>(c.uo, oo, 0);                                                             \
+        um[r] = buf_ld<T>(INC ? c.du : c.uo, oo, 0);                                                \
         const int mp = clampi(xs - c.mx_off, c.mx_lo, c.mx_hi) * c.mplane;                         \
         cv[r] = buf_ld<T>(c.model, mv, mp);                                                         \
         T f_ = c.has_rhs ? buf_ld<T>(c.rhs, oo, 0) : T(0);                                          \
@@ -315,17 +332,7 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
         const int xp = (inx ? x : nx - 1) * pl;
         T f = fv[j];
         if (MODE == 3) f += mm[j] * (-dv[j]);
-        T lx = T(0), lyy = T(0), lz = T(0);
-#pragma unroll
-        for (int i = R; i >= 1; --i) {
-            lx = fma(P.cx.d2[i], q[R + j + i] + q[R + j - i], lx);
-            lyy = fma(P.cy.d2[i], lu[own + i * LZ] + lu[own - i * LZ], lyy);
-            lz = fma(P.cz.d2[i], lu[own + i] + lu[own - i], lz);
-        }
-        lx = fma(P.cx.d2[0], uc, lx);
-        lyy = fma(P.cy.d2[0], uc, lyy);
-        lz = fma(P.cz.d2[0], uc, lz);
-        const T lap = (lx + lyy) + lz;
+        const T lap = laplacian3<T, R, LZ>(P, q + j, lu, own, uc);
         const T c2 = cv[j] * cv[j];
         const T u_m = um[j];
         const bool edge = XC ? c.yzedge : ((x == 0) | (x == nx - 1) | c.yzedge);
@@ -381,17 +388,30 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
             dws = s3m * psc;
         }
         const T fac1 = P.tc.two_dt2 / (T(2) + dt * s1k);
-        const T fac2 = c2 * srcp - (u_m - T(2) * uc) * P.tc.inv_dt2 + s1k * u_m * P.tc.inv_2dt - s2k * uc;
-        const T upd = fac1 * fac2;
-        const T upi = fma(P.tc.dt2, c2 * src, T(2) * uc - u_m);
+        const T S = c2 * srcp;
         T pxn = pml ? pxd : pxc;
         T pyn = pml ? pyd : pycj;
         T pzn = pml ? pzd : pzcj;
-        T up = pml ? upd : upi;
-        T dw = c2 * srcp + (s1m - s1k) * (up - u_m) * P.tc.inv_2dt + (s2m - s2k) * uc;
+        T up, dn = T(0), dw;
+        if constexpr (INC) {
+            // du_{k+1} = du_k + fac1 (S - s2 u - s1 du_k / dt): the reference's
+            // damped update with u_{k-1} = u_k - du_k; undamped du_k + dt^2 S
+            const T dnd = u_m + fac1 * ((S - s2k * uc) - s1k * u_m * P.tc.inv_dt);
+            const T dni = fma(P.tc.dt2, c2 * src, u_m);
+            dn = pml ? dnd : dni;
+            up = uc + dn;
+            dw = S + (s1m - s1k) * (dn + u_m) * P.tc.inv_2dt + (s2m - s2k) * uc;
+        } else {
+            const T fac2 = S - (u_m - T(2) * uc) * P.tc.inv_dt2 + s1k * u_m * P.tc.inv_2dt - s2k * uc;
+            const T upd = fac1 * fac2;
+            const T upi = fma(P.tc.dt2, c2 * src, T(2) * uc - u_m);
+            up = pml ? upd : upi;
+            dw = S + (s1m - s1k) * (up - u_m) * P.tc.inv_2dt + (s2m - s2k) * uc;
+        }
         if constexpr (!XC) dw = dw + dws;
         if (edge) {
             up = T(0);
+            dn = T(0);
             dw = T(0);
             pxn = T(0);
             pyn = T(0);
@@ -405,6 +425,7 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
             buf_st(c.ic, inx ? at(ovi, xp * (8 / SZ)) : kOOB, c.shb8, icv[j] + (double)uc * (double)dv[j]);
         if constexpr (MODE == 4) buf_st(c.ic, inx ? at(ovi, xp) : kOOB, c.shb, icv[j] + uc * dv[j]);
         buf_st(c.uo, oo, 0, up);
+        if constexpr (INC) buf_st(c.du, oo, 0, dn);
         buf_st(c.pxo, op, 0, pxn);
         buf_st(c.pyo, op, 0, pyn);
         buf_st(c.pzo, op, 0, pzn);
@@ -416,7 +437,7 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
 #undef PA_ROW
 }
 
-template <typename T, int R, int MODE>
+template <typename T, int R, int MODE, bool INC>
 __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DParams<T> &P,
                                        const pysit_amd_shot &S, int tile, int full, int xclear, int tloc, int slot,
                                        int seg_base, Lds3D<T, R> &L)
@@ -478,6 +499,7 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
     const T *vb = (const T *)B.u[0];
     c.u = make_rsrc((const T *)B.u[cur] + off, nb);
     c.uo = make_rsrc((T *)B.u[cur ^ 1] + off, nb);
+    c.du = make_rsrc(INC ? (const void *)((T *)B.du + off) : (const void *)vb, INC ? nb : 0);
     c.px = make_rsrc((const T *)B.phix[cur] + off, full ? nb : 0);
     c.pxo = make_rsrc((T *)B.phix[cur ^ 1] + off, full ? nb : 0);
     c.py = make_rsrc((const T *)B.phiy[cur] + off, full ? nb : 0);
@@ -559,9 +581,9 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
     }
     if (!has_inj && threadIdx.x == 0) L.junk[3][0] = T(0);
     lds_barrier();
-    if (full && xclear) march3_full<T, R, MODE, true>(c, P, L, x0, hl_pz);
-    else if (full) march3_full<T, R, MODE, false>(c, P, L, x0, hl_pz);
-    else march3_int<T, R, MODE>(c, P, L, x0);
+    if (full && xclear) march3_full<T, R, MODE, true, INC>(c, P, L, x0, hl_pz);
+    else if (full) march3_full<T, R, MODE, false, INC>(c, P, L, x0, hl_pz);
+    else march3_int<T, R, MODE, INC>(c, P, L, x0);
 }
 
 // Receiver sampling, 16 lanes per receiver with a fixed reduction order
@@ -589,7 +611,9 @@ __device__ __forceinline__ void sample_group3(const pysit_amd_batch &B, int slot
 #ifndef PA3D_WAVES_PER_EU
 #define PA3D_WAVES_PER_EU 4
 #endif
-template <typename T, int R, int MODE>
+// INC: the batch's increment form (du); false: the reference-contract step
+// (u_{k-1} in the second u buffer)
+template <typename T, int R, int MODE, bool INC>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? PA3D_WAVES_PER_EU : 2)))
 void k_step3d(const pysit_amd_batch B, const Step3DParams<T> P,
                                                 int slot, int seg_base, int n_tile_blocks,
@@ -611,11 +635,11 @@ void k_step3d(const pysit_amd_batch B, const Step3DParams<T> P,
     }
     const int blk = xcd_block(b, tile_blocks);
     if (blk >= n_tile_blocks) return;
-    if (P.single) {
-        tile3d<T, R, MODE>(B, P, P.shot0, blk, 1, 0, blk, slot, seg_base, L);
+    if (!INC) {
+        tile3d<T, R, MODE, INC>(B, P, P.shot0, blk, 1, 0, blk, slot, seg_base, L);
     } else {
         const int4 td = reinterpret_cast<const int4 *>(B.tiles)[blk];
-        tile3d<T, R, MODE>(B, P, B.shots[td.x], blk, td.z & 1, (td.z >> 1) & 1, td.y, slot, seg_base, L);
+        tile3d<T, R, MODE, INC>(B, P, B.shots[td.x], blk, td.z & 1, (td.z >> 1) & 1, td.y, slot, seg_base, L);
     }
 }
 
@@ -627,8 +651,18 @@ static int launch3d(const pysit_amd_batch &B, const Step3DParams<T> &P, int slot
     const int rec_blocks = recs ? (recs + TY3 * 4 - 1) / (TY3 * 4) : 0;
     const int blocks = n_tiles + rec_blocks;
     if (blocks == 0) return PYSIT_AMD_OK;
-    hipLaunchKernelGGL((k_step3d<T, R, MODE>), dim3(blocks), dim3(TY3 * 64), 0, st, B, P, slot,
-                       seg_base, n_tiles, recs, rec_blocks, cursor);
+    if (P.single) {
+        if constexpr (MODE == 0) {
+            hipLaunchKernelGGL((k_step3d<T, R, MODE, false>), dim3(blocks), dim3(TY3 * 64), 0, st, B, P, slot,
+                               seg_base, n_tiles, recs, rec_blocks, cursor);
+        } else {
+            set_error("the single-shot step only propagates");
+            return PYSIT_AMD_EINVAL;
+        }
+    } else {
+        hipLaunchKernelGGL((k_step3d<T, R, MODE, true>), dim3(blocks), dim3(TY3 * 64), 0, st, B, P, slot,
+                           seg_base, n_tiles, recs, rec_blocks, cursor);
+    }
     PA_HIP(hipGetLastError());
     return PYSIT_AMD_OK;
 }
@@ -684,6 +718,9 @@ int run3d(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int 
     // every per-batch array (the FP64 accumulator at 8 B per node the
     // largest) is reached through 32-bit buffer offsets
     PA_CHECK(B.total_n * 8 <= kMaxRsrcBytes, "batch too large for one launch (total_n)");
+    PA_CHECK(B.u[0] && B.u[1] && B.du && B.phix[0] && B.phix[1] && B.phiy[0] && B.phiy[1] && B.phiz[0] &&
+                 B.phiz[1] && B.psi[0] && B.psi[1],
+             "3D batch state buffers (u, phix, phiy, phiz, psi, du) must all be given");
     Step3DParams<T> P{};
     const int r = B.order / 2;
     P.cx = axis_coef<T>(r, B.dx);

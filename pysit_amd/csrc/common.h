@@ -68,7 +68,7 @@ inline AxisCoef<T> axis_coef(int r, double h)
 
 template <typename T>
 struct TimeCoef {
-    T dt, dt2, inv_dt2, inv_2dt, two_dt2;
+    T dt, dt2, inv_dt2, inv_2dt, two_dt2, inv_dt;
 };
 
 template <typename T>
@@ -80,6 +80,7 @@ inline TimeCoef<T> time_coef(double dt)
     c.inv_dt2 = (T)(1.0 / (dt * dt));
     c.inv_2dt = (T)(1.0 / (2.0 * dt));
     c.two_dt2 = (T)(2.0 * dt * dt);
+    c.inv_dt = (T)(1.0 / dt);
     return c;
 }
 

@@ -48,9 +48,9 @@ MAX_REGION_BYTES = 0x7FFF0000
 
 def max_batch_nodes(dim, itemsize):
     """Most padded nodes one batch may hold: the FP64 imaging accumulator
-    (8 B per node) and, in 2D, the six state buffers, kept in one region,
+    (8 B per node) and, in 2D, the seven state buffers, kept in one region,
     must each fit one buffer resource."""
-    return MAX_REGION_BYTES // (max(8, 6 * itemsize) if dim == 2 else 8)
+    return MAX_REGION_BYTES // (max(8, 7 * itemsize) if dim == 2 else 8)
 
 
 def shot_nodes(shape, itemsize):
@@ -161,13 +161,24 @@ class DeviceBatch:
     precompute_injection = True
 
     def __init__(self, shots, model, deltas, dt, order, dtype=np.float32, tile_x=None,
-                 imaging_period=1, memory_budget=None, ic_window=None, tile_order="shot", graph_steps=None):
+                 imaging_period=1, memory_budget=None, ic_window=None, tile_order="shot", graph_steps=None,
+                 adjoint="pysit"):
         if not shots:
             raise ValueError("empty shot batch")
-        self.dev = _dev()
         self.dim = shots[0].dim
         if any(s.dim != self.dim for s in shots):
             raise ValueError("all shots of a batch share the dimension")
+        # adjoint: "pysit" stores PySIT's dWaveOp (mesh PML profile), the
+        # imaging factor of migrate_shot; "exact" stores the step's own
+        # derivative with respect to m (kernel profile), with which the
+        # adjoint recursion gives the exact gradient of the discrete misfit
+        # (2D; in 3D only without PML, where the two agree)
+        if adjoint not in ("pysit", "exact"):
+            raise ValueError("adjoint must be 'pysit' or 'exact'")
+        if adjoint == "exact" and self.dim == 3 and any(l.size or r.size for s in shots for l, r in s.kpml):
+            raise ValueError("the exact adjoint is built for 2D; 3D batches with PML use adjoint='pysit'")
+        self.adjoint = adjoint
+        self.dev = _dev()
         if order not in (2, 4, 6, 8):
             raise ValueError("spatial_accuracy_order must be 2, 4, 6 or 8")
         self.order = int(order)
@@ -532,13 +543,19 @@ class DeviceBatch:
         return ("u", "phix", "phiz") if self.dim == 2 else ("u", "phix", "phiy", "phiz", "psi")
 
     def _new_state(self):
-        """Both time levels of every field, in one allocation (the 2D kernel
-        reaches all of them through one buffer resource)."""
+        """Both time levels of every field and the increment du, in one
+        allocation (the 2D kernel reaches all of them through one buffer
+        resource).  At rest every buffer is zero."""
         fields = self._fields()
-        block = torch.zeros((2 * len(fields), self.total_n), dtype=self.tdtype, device=self.dev)
-        return {f: block[2 * i:2 * i + 2] for i, f in enumerate(fields)}
+        block = torch.zeros((2 * len(fields) + 1, self.total_n), dtype=self.tdtype, device=self.dev)
+        st = {f: block[2 * i:2 * i + 2] for i, f in enumerate(fields)}
+        st["du"] = block[2 * len(fields):]
+        return st
 
     def _alloc_state(self):
+        # forward() calls so far: a consumer of the stores (a pending
+        # backward) checks that no later forward replaced them
+        self.generation = 0
         self.fstate = self._new_state()
         self.qstate = None
         self.dw = None
@@ -546,7 +563,7 @@ class DeviceBatch:
         self.ic = None
 
     def state_bytes(self):
-        return len(self._fields()) * 2 * self.total_n * self.dtype.itemsize
+        return (len(self._fields()) * 2 + 1) * self.total_n * self.dtype.itemsize
 
     def _batch(self, state, adjoint, ic=None, dw=None):
         b = N.Batch()
@@ -574,7 +591,9 @@ class DeviceBatch:
             if f in state:
                 arr[0] = state[f][0].data_ptr()
                 arr[1] = state[f][1].data_ptr()
+        b.du = state["du"].data_ptr()
         b.total_n = self.total_n
+        b.dw_form = 1 if (self.adjoint == "exact" and self.dim == 2) else 0
         b.dw = dw.data_ptr() if dw is not None else None
         b.ic = ic.data_ptr() if ic is not None else None
         b.imaging_period = self.imaging_period
@@ -621,6 +640,9 @@ class DeviceBatch:
         n_inj = int(self.adj_inj[2].numel()) - 1 if self.adj_inj is not None else 0
         per_slot = (self.total_n + n_inj) * item
         ckpt = self.state_bytes()
+        # what migrate allocates beside the store: the adjoint state (unless
+        # held already) and the FP64 and FP32 imaging accumulators
+        budget -= (0 if getattr(self, "qstate", None) is not None else ckpt) + 12 * self.total_n
         if self.max_nt * per_slot <= budget:
             return self.max_nt
         # nseg segments: nseg checkpoints, one working state, a K-slot store
@@ -652,6 +674,7 @@ class DeviceBatch:
         at segment boundaries.  ``store_field`` also keeps u_k of every step
         (``field_store``).  Returns the per-shot traces (device FP64, in the
         caller's shot order)."""
+        self.generation += 1
         for t in self.traces:
             t.zero_()
         for f in self.fstate.values():
@@ -668,6 +691,13 @@ class DeviceBatch:
         elif keep_dw:
             if store_field:
                 raise ValueError("storing the wavefield needs the whole dWaveOp history in memory")
+            if self.dw is not None and self.dw_slots > K:
+                # the budget counted a larger store held from an earlier call
+                # as available: release it before the checkpoints are taken
+                self.dw = None
+                self.dw_slots = 0
+                self._inj_pre = None
+                torch.cuda.empty_cache()
             b = self._batch(self.fstate, False)
             for j0 in range(0, self.max_nt, K):
                 self.ckpts.append({f: v.clone() for f, v in self.fstate.items()})
@@ -688,6 +718,7 @@ class DeviceBatch:
         """Use host-provided dWaveOp (per shot, caller order: a sequence of
         padded arrays, one per imaged step) as the stored history."""
         p = self.imaging_period
+        self.generation += 1
         self._ensure_dw(self.max_nt)
         self.dw.zero_()
         for pos, orig in enumerate(self.order_idx):

@@ -56,7 +56,7 @@ class TemporalModeling(object):
     solver_type = "time"
     modeling_type = "time"
 
-    def __init__(self, solver, memory_budget=None, tile_x=None, ic_window=None, autotune=True):
+    def __init__(self, solver, memory_budget=None, tile_x=None, ic_window=None, autotune=True, adjoint="pysit"):
         if self.solver_type != solver.supports["equation_dynamics"]:
             raise TypeError(f"Argument 'solver' type {solver.supports['equation_dynamics']} does not match "
                             f"modeling solver type {self.solver_type}.")
@@ -67,6 +67,14 @@ class TemporalModeling(object):
         # launch configuration (tile rows, tile order, graphs) picked by timing
         # the candidates once per batch signature, unless tile_x is given
         self.autotune = bool(autotune)
+        # "pysit": dWaveOp as the reference forms it (mesh PML profile), so
+        # migrate_shot and the Born operator are PySIT's; "exact": the step's
+        # own m-derivative (kernel PML profile), so migrate_shot is the exact
+        # adjoint of the discrete forward map and linear_forward_model its
+        # exact linearisation (2D; see DeviceBatch)
+        if adjoint not in ("pysit", "exact"):
+            raise ValueError("adjoint must be 'pysit' or 'exact'")
+        self.adjoint = adjoint
 
     # ---------------------------------------------------------------- plumbing
     def geometry(self, shot):
@@ -100,7 +108,7 @@ class TemporalModeling(object):
         history."""
         s = self.solver
         key = (tuple(id(sh) for sh in shots), s.dt, s.nsteps, imaging_period, np.dtype(s.dtype).str,
-               s.spatial_accuracy_order, tuple(s.mesh.shape(include_bc=True, as_grid=True)))
+               s.spatial_accuracy_order, tuple(s.mesh.shape(include_bc=True, as_grid=True)), self.adjoint)
         cached = getattr(self, "_batch_cache", None)
         if reuse and cached is not None and cached[0] == key:
             b = cached[1]
@@ -119,7 +127,7 @@ class TemporalModeling(object):
         unless disabled or tile_x is fixed."""
         s = self.solver
         kw = dict(dtype=s.dtype, imaging_period=imaging_period, memory_budget=self.memory_budget,
-                  ic_window=self.ic_window)
+                  ic_window=self.ic_window, adjoint=self.adjoint)
         if self.autotune and not self.tile_x:
             from ..autotune import tuned_batch
             return tuned_batch(geometries, model, s.mesh.deltas, s.dt, s.spatial_accuracy_order, **kw)
@@ -179,9 +187,9 @@ class TemporalModeling(object):
             h = dWaveOp[0]
             b = h.batch
             if (b.dt == s.dt and b.max_nt == s.nsteps and b.imaging_period == imaging_period
-                    and len(dWaveOp) == len(range(0, s.nsteps, imaging_period))):
+                    and b.adjoint == self.adjoint and len(dWaveOp) == len(range(0, s.nsteps, imaging_period))):
                 return b, h.shot
-            raise ValueError("dWaveOp handles belong to another model, time axis or imaging period")
+            raise ValueError("dWaveOp handles belong to another model, time axis, imaging period or adjoint mode")
         b = self.batch([shot], imaging_period)
         if dWaveOp is not None:
             b.load_dw([list(dWaveOp)])

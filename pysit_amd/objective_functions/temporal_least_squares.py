@@ -34,13 +34,23 @@ class ObservedOnDevice(object):
     (interp1d._call_linear: bracketing samples by searchsorted clipped to
     [1, n-1], slope (y_hi - y_lo) / (x_hi - x_lo), value slope (t - x_lo) +
     y_lo; zero outside [x_0, x_-1]).  Receivers with a time window take the
-    host code.  The data are uploaded once per shot (again when
-    ``receivers.data`` is replaced); the bracketing indices are kept per time
-    axis."""
+    host code.  The data are uploaded once per shot and again when they
+    change through the receivers (``receivers.data = ...`` or sampling into
+    them, which bump ``receivers.data_version``).  An edit of the array in
+    place (``receivers.data += noise``) is not seen: call :meth:`invalidate`
+    after one.  The bracketing indices are kept per time axis."""
 
     def __init__(self):
         self._data = {}
         self._plans = {}
+
+    def invalidate(self, shot=None):
+        """Forget the device copy of ``shot``'s observed data (all shots if
+        None), so the next call uploads the current array."""
+        if shot is None:
+            self._data.clear()
+        else:
+            self._data.pop(id(shot), None)
 
     def __call__(self, shot, ts, dev):
         import torch
@@ -50,9 +60,10 @@ class ObservedOnDevice(object):
             # code does (receivers.py:95-96): take that path itself
             return torch.as_tensor(np.asarray(rec.interpolate_data(ts), dtype=np.float64), device=dev)
         d = rec.data
+        ver = getattr(rec, "data_version", 0)
         ent = self._data.get(id(shot))
-        if ent is None or ent[0] is not d or ent[1] is not shot:
-            ent = (d, shot, torch.as_tensor(np.ascontiguousarray(d, dtype=np.float64), device=dev))
+        if ent is None or ent[0] is not d or ent[1] is not shot or ent[3] != ver:
+            ent = (d, shot, torch.as_tensor(np.ascontiguousarray(d, dtype=np.float64), device=dev), ver)
             self._data[id(shot)] = ent
         y = ent[2]
         x = np.asarray(rec.ts, dtype=np.float64)

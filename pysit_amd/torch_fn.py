@@ -9,13 +9,25 @@ condition against the stored (or checkpoint-recomputed) dWaveOp gives
 
     dL/dm = -prod(deltas) * fold( sum_k q_k[G] * dWaveOp_k ),
 
-the transpose of the Born operator of
-pysit/modeling/temporal_modeling.py:389-546 with PySIT's adjoint
-(:228-387): exact without PML; with PML it carries the reference's forward-PML
-adjoint, i.e. it equals PySIT's migrate_shot up to the factor
--prod(deltas)/dt (``TemporalLeastSquares`` gradients are reproduced by
-``L = 0.5 * dt * ||d - d_obs||^2``).  dt is fixed by the model given to the
-forward (CFL), as in PySIT.
+with PySIT's adjoint recursion (pysit/modeling/temporal_modeling.py:228-387).
+``ShotProblem(adjoint=...)`` picks the dWaveOp it images against:
+
+* ``"pysit"`` (default): PySIT's, built with the mesh PML profile
+  (constant_density_acoustic_base.py:24-40).  The backward then equals
+  PySIT's migrate_shot up to the factor -prod(deltas)/dt
+  (``TemporalLeastSquares`` gradients are reproduced by
+  ``L = 0.5 * dt * ||d - d_obs||^2``); it is the exact gradient without PML
+  and departs from it on the right-hand PML sides, where the reference's
+  kernel profile sits one node off its mesh profile.
+* ``"exact"`` (2D): the step's own derivative with respect to m (kernel
+  profile), which makes the backward the exact gradient of the discrete
+  forward map with the PML on as well: the adjoint recursion is the step's
+  transpose after the change of variables Phi' = -dt (sz - sx) mu of the
+  auxiliary multipliers.
+
+dt is fixed by the model given to the forward (CFL), as in PySIT.  A batch
+serves one pending backward: a second forward on the same problem before
+the first output's backward makes that backward raise.
 """
 import numpy as np
 import torch
@@ -30,12 +42,12 @@ class ShotProblem(object):
     per model change by ``TemporalModeling.geometry``)."""
 
     def __init__(self, solver, shots, imaging_period=1, memory_budget=None, tile_x=None, segment=None,
-                 ic_window=None, autotune=True):
+                 ic_window=None, autotune=True, adjoint="pysit"):
         from .modeling.temporal_modeling import TemporalModeling
         self.solver = solver
         self.shots = shots
         self.tm = TemporalModeling(solver, memory_budget=memory_budget, tile_x=tile_x, ic_window=ic_window,
-                                   autotune=autotune)
+                                   autotune=autotune, adjoint=adjoint)
         self.imaging_period = imaging_period
         self.segment = segment
         self.batch = None
@@ -80,6 +92,7 @@ class ShotTraces(torch.autograd.Function):
         tr = b.forward(keep_dw=True, segment=problem.segment)
         ctx.problem = problem
         ctx.batch = b
+        ctx.generation = b.generation
         ctx.m_shape = m.shape
         return torch.stack(tr)
 
@@ -87,6 +100,12 @@ class ShotTraces(torch.autograd.Function):
     def backward(ctx, grad_traces):
         b = ctx.batch
         p = ctx.problem
+        if b.generation != ctx.generation:
+            # the batch's model, dWaveOp store and checkpoints now belong to a
+            # later forward: this backward would image against that one
+            raise RuntimeError("ShotTraces: the shot batch ran another forward since this output was "
+                               "computed; call backward before the next forward on the same ShotProblem, "
+                               "or use one ShotProblem per pending graph")
         mesh = p.solver.mesh
         b.migrate([grad_traces[i].contiguous() for i in range(b.n_shots)])
         out = torch.zeros(tuple(mesh.shape(as_grid=True)), dtype=torch.float64, device=b.dev)

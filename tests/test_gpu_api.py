@@ -180,7 +180,7 @@ def test_default_batches_split_at_the_addressable_region(gpu, monkeypatch):
 
     shape = tuple(solver.mesh.shape(include_bc=True, as_grid=True))
     n = device.shot_nodes(shape, 8)
-    monkeypatch.setattr(device, "MAX_REGION_BYTES", 48 * n)  # six FP64 state buffers of one shot
+    monkeypatch.setattr(device, "MAX_REGION_BYTES", 56 * n)  # seven FP64 state buffers of one shot
     assert device.shots_per_batch(shape, 8) == 1
     split = copy.deepcopy(shots)
     pa.generate_seismic_data(split, solver, model)

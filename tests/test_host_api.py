@@ -341,6 +341,7 @@ def test_batch_limits_follow_the_kernels_addressable_region():
     n2 = device.shot_nodes((1041, 341), 4)
     assert n2 == 1041 * 352                                         # rows at 128 B
     k2 = device.shots_per_batch((1041, 341), 4)
-    assert k2 * n2 * 24 <= lim < (k2 + 1) * n2 * 24                 # six FP32 state buffers
-    assert device.shots_per_batch((1041, 341), 8) == k2 // 2
+    assert k2 * n2 * 28 <= lim < (k2 + 1) * n2 * 28                 # seven FP32 state buffers
+    k2d = device.shots_per_batch((1041, 341), 8)
+    assert k2d * n2 * 56 <= lim < (k2d + 1) * n2 * 56
     assert device.shots_per_batch((2000, 2000, 2000), 4) == 1
