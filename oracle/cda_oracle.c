@@ -23,6 +23,11 @@
  *                      as storing every step (the loop is deterministic).
  *   orc_fold           pysit/solvers/model_parameter.py:537-593 (add_padding)
  *
+ * The node loops of the steps, dWaveOp and the imaging condition run over
+ * OpenMP threads when built with -fopenmp: every node is computed by one
+ * thread with the same operations, so the results do not depend on the
+ * thread count.
+ *
  * Floating point follows the reference's source literally: the tap weights are
  * double literals, the taps are summed left to right in double and then
  * rounded to T, and every update expression keeps the reference's evaluation
@@ -135,6 +140,7 @@ ORC_DEFINE_TAPS(float, f)
         const T hx = (T)dx, hz = (T)dz;                                           \
         const T hx2 = hx * hx, hz2 = hz * hz;                                     \
         const T dt2 = (T)(dt * dt);                                               \
+        _Pragma("omp parallel for schedule(static)")                              \
         for (int i = 0; i < nx; ++i) {                                            \
             for (int k = 0; k < nz; ++k) {                                        \
                 const long idx = (long)i * nz + k;                                \
@@ -195,6 +201,7 @@ ORC_DEFINE_STEP2D(float, f)
         const T hx2 = hx * hx, hy2 = hy * hy, hz2 = hz * hz;                      \
         const T dt2 = (T)(dt * dt);                                               \
         const long sxs = (long)ny * nz, sys = nz;                                 \
+        _Pragma("omp parallel for schedule(static)")                              \
         for (int i = 0; i < nx; ++i) {                                            \
             for (int j = 0; j < ny; ++j) {                                        \
                 for (int k = 0; k < nz; ++k) {                                    \
@@ -354,6 +361,7 @@ static void orc_dwaveop(const orc_grid *g, const orc_level *km1, const orc_level
                         const orc_level *kp1, double *out)
 {
     const double dt2 = g->dt * g->dt, tdt = 2.0 * g->dt;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < g->nx; ++i)
         for (int j = 0; j < g->ny; ++j)
             for (int q = 0; q < g->nz; ++q) {
@@ -551,6 +559,7 @@ ORC_EXPORT int orc_migrate(int dim, int order, int nx, int ny, int nz, double dx
         for (int s = k1 - 1; s >= k0 && rc == 0; --s) {
             if (s % p == 0) {
                 const double *dw = seg_dw + (size_t)(s - k0) * N;
+#pragma omp parallel for schedule(static)
                 for (long i = 0; i < N; ++i) ic_padded[i] += qk->u[i] * dw[i];
             }
             orc_inject(rhs, n_adj, adj_idx, adj_chan, adj_w,

@@ -288,8 +288,8 @@ static int batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int slo
     PA_CHECK(B.tile_x >= 1 && (B.dim == 2 || B.tile_y >= 1), "tile sizes must be positive");
     PA_CHECK(B.n_shots >= 1 && B.shots && B.n_tiles >= 0 && (B.n_tiles == 0 || B.tiles),
              "shot or tile table missing");
-    PA_CHECK(B.u[0] && B.u[1] && B.phix[0] && B.phix[1] && B.phiz[0] && B.phiz[1],
-             "state buffers missing");
+    PA_CHECK(B.u[0] && B.u[1] && B.phix[0] && B.phix[1] && B.phiz[0] && B.phiz[1] && B.du,
+             "state buffers missing (u, phix, phiz and du)");
     PA_CHECK(B.dim == 2 || (B.phiy[0] && B.phiy[1] && B.psi[0] && B.psi[1]),
              "3D state buffers missing");
     PA_CHECK(B.imaging_period >= 1, "imaging_period must be >= 1");

@@ -182,3 +182,32 @@ def rel_l2(a, b):
     b = np.asarray(b, dtype=np.float64)
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def oracle_from_geometry(g, model, deltas, dt, order):
+    """The oracle Problem of one device ShotGeometry over ``model`` (the
+    unpadded wave speed the geometry's window reads): the edge-padded C by
+    the window's clamping, the mesh sigma laid out from the kernel arrays
+    (left by node, right on the last nodes), flat entry lists, the adjoint
+    entries in padded coordinates."""
+    from oracle.oracle import Problem
+    model = np.asarray(model, dtype=np.float64)
+    idx = [np.clip(np.arange(n) - off, lo, hi) for n, (off, lo, hi) in zip(g.shape, g.window)]
+    Cpad = model[np.ix_(*idx)]
+    msig = [mesh_sigma(n, l, r) for n, (l, r) in zip(g.shape, g.kpml)]
+    pos, chan, w = g.src
+    src = (np.ravel_multi_index(tuple(pos.T), g.shape), chan, w)
+    apos, achan, aw = g.adj
+    adj = (np.ravel_multi_index(tuple(apos.T), g.shape), achan, aw)
+    pads = []
+    for n, (off, lo, hi) in zip(g.shape, g.window):
+        left = off + lo
+        pads.append((left, n - left - (hi - lo + 1)))
+    return Problem(g.shape, deltas, dt, g.nt, Cpad, g.kpml, msig, src, g.amp[:g.nt], g.rec, g.nrec, adj=adj,
+                   order=order, pad=pads)
+
+
+def truncated_geometry(g, nt):
+    """The same shot with only its first ``nt`` steps."""
+    from pysit_amd.device import ShotGeometry
+    return ShotGeometry(g.shape, g.kpml, g.window, nt, g.src, g.amp[:nt], g.rec, g.nrec, adj=g.adj)

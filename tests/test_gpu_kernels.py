@@ -58,12 +58,14 @@ def test_step2d_matches_oracle(gpu, dtype, order, shape):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("order", [2, 8])
+@pytest.mark.parametrize("order", [2, 4, 6, 8])
 def test_step3d_matches_oracle(gpu, dtype, order):
+    # nx = 37: two 16-plane x tiles and a partial third, so the step crosses
+    # x-tile boundaries at every order
     import torch
     from pysit_amd import _native as N
     rng = np.random.default_rng(7 + order)
-    nx, ny, nz = 13, 17, 71
+    nx, ny, nz = 37, 17, 71
     n = nx * ny * nz
     f = [rng.standard_normal(n) for _ in range(8)]
     f[6] = 1.0 + 0.5 * rng.random(n)
@@ -161,20 +163,21 @@ def test_batch_fp32_close_to_fp64_oracle(gpu):
     tr = b.forward(keep_dw=True)[0].cpu().numpy()
     op = p.oracle_problem()
     ref = op.forward()["simdata"]
-    assert rel_l2(tr, ref) < 1e-4
     rng = np.random.default_rng(2)
     res = [ref - 0.9 * ref + 0.01 * rng.standard_normal(ref.shape) * np.abs(ref).max()]
     b.migrate(res)
     img = b.image(0).cpu().numpy().reshape(-1) * (-p.dt)
     img_ref = op.migrate(res[0], fold=False)
-    assert rel_l2(img, img_ref) < 1e-3
+    e_tr, e_img = rel_l2(tr, ref), rel_l2(img, img_ref)
+    print(f"FP32, 400 steps: traces {e_tr:.2e}, gradient {e_img:.2e}")
+    assert e_tr < 1e-5 and e_img < 1e-4
 
 
 def test_fp32_accuracy_over_a_long_run(gpu):
-    # 2000 steps in FP32 against the FP64 oracle: the error grows with the
-    # step count like the reference's own FP32 arithmetic does (about 1.9e-4
-    # on a 3000-step shot of the 1041 x 341 layered case).  Here it measures
-    # 1.5e-5 (traces) and 9e-6 (gradient).
+    # 2000 steps in FP32 against the FP64 oracle.  The batch steps evaluate
+    # the Laplacian with difference-form taps and the update in increment
+    # form: 2.3e-6 (traces) and 1.8e-6 (gradient) here, where the plain form
+    # (the reference's own FP32 arithmetic) gave 1.5e-5 and 9e-6.
     p = Synthetic((90, 110), ((10, 10), (10, 10)), (0.01, 0.01), 2000, 8, seed=4)
     b = _batch([p], np.float32)
     tr = b.forward(keep_dw=True)[0].cpu().numpy()
@@ -187,7 +190,7 @@ def test_fp32_accuracy_over_a_long_run(gpu):
     img_ref = op.migrate(res[0], fold=False)
     e_tr, e_img = rel_l2(tr, ref), rel_l2(img, img_ref)
     print(f"FP32, 2000 steps: traces {e_tr:.2e}, gradient {e_img:.2e}")
-    assert e_tr < 3e-5 and e_img < 3e-5, (e_tr, e_img)
+    assert e_tr < 1e-5 and e_img < 1e-5, (e_tr, e_img)
 
 
 def test_ic_window_accumulator_matches_fp64_accumulation(gpu):
@@ -227,9 +230,10 @@ def test_ic_window_accumulator_3d_matches_fp64_accumulation(gpu):
         assert rel_l2(img, ref) < 1e-5
 
 
-@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-10), (np.float32, 1e-3)])
-def test_batch_3d(gpu, dtype, tol):
-    p = Synthetic((20, 22, 70), ((4, 4), (3, 5), (4, 3)), (0.01, 0.011, 0.012), 90, 8, seed=5, n_rec=5)
+@pytest.mark.parametrize("dtype,tol,order", [(np.float64, 1e-10, 4), (np.float64, 1e-10, 6), (np.float64, 1e-10, 8),
+                                             (np.float32, 1e-5, 8)])
+def test_batch_3d(gpu, dtype, tol, order):
+    p = Synthetic((20, 22, 70), ((4, 4), (3, 5), (4, 3)), (0.01, 0.011, 0.012), 90, order, seed=5, n_rec=5)
     b = _batch([p], dtype)
     tr = b.forward(keep_dw=True)[0].cpu().numpy()
     op = p.oracle_problem()
@@ -243,7 +247,7 @@ def test_batch_3d(gpu, dtype, tol):
     assert rel_l2(img, img_ref) < tol * 10
 
 
-@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-10), (np.float32, 1e-3)])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-10), (np.float32, 1e-5)])
 def test_batch_3d_with_plain_tiles(gpu, dtype, tol):
     # 64 x 36 x 140 nodes with 4-node PML: the x-y tiles in the middle (x
     # tiles of 16 planes, y tiles of 8 rows) and the z tile at 64..127 are
@@ -287,7 +291,7 @@ def test_layered_3d_workload_is_stable_and_fp32_follows_fp64(gpu):
     assert np.all(np.isfinite(t64)) and np.all(np.isfinite(t32))
     late, peak = np.abs(t64[-100:]).max(), np.abs(t64).max()
     assert late < peak
-    assert rel_l2(t32, t64) < 1e-4
+    assert rel_l2(t32, t64) < 1e-5
 
 
 def test_fold_matches_oracle(gpu):
