@@ -245,6 +245,25 @@ int pysit_amd_batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int 
                         int direction, int seg_base, const int *active_tiles,
                         const int *active_recs, void *stream);
 
+/* pysit_amd_batch_run without a caller workspace.  No counterpart in the
+ * reference: this entry point is for callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid allocation).  It allocates nothing: no hipMalloc, hipFree or
+ * allocating library routine.  Same arguments and results as
+ * pysit_amd_batch_run (bit for bit), except:
+ *   - b->ic32 and b->inj_pre must be NULL (no workspace is taken); the steps
+ *     gather their sources themselves, which gives the same values;
+ *   - the FP32 window accumulator of an FP32 imaging run (mode 2,
+ *     ic_window >= 1, ic and dw given) lives in the library's static device
+ *     array of 64 MiB per GPU, so such a run holds at most 16 Mi nodes
+ *     (total_n); larger batches are refused with PYSIT_AMD_EINVAL;
+ *   - every call on one GPU shares that GPU's array: calls that overlap on
+ *     one GPU (two streams, two threads) race.  Each call clears the part it
+ *     uses before its first step (with a kernel, not a memset).           */
+int pysit_amd_batch_run_static(const pysit_amd_batch *b, int mode, int slot_begin, int slot_end,
+                               int direction, int seg_base, const int *active_tiles,
+                               const int *active_recs, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Plans: the batch time loop as replayed hipGraphs.                         */
 /*                                                                           */

@@ -3,6 +3,8 @@
 // error strings, and the PML fold kernel.
 #include "common.h"
 
+#include <mutex>
+
 namespace pa {
 
 static thread_local std::string g_err;
@@ -329,6 +331,46 @@ static int batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int slo
 
 int plan_device(const pysit_amd_plan *plan);
 
+// ---- the run without a caller workspace (pysit_amd_batch_run_static) ------
+// The FP32 window accumulator of an adjoint run is partial sums: scratch
+// memory beyond the caller's inputs and outputs.  pysit_amd_batch_run takes it
+// from the caller (b->ic32); the static run keeps it in this file-scope
+// device array instead, so it allocates nothing.  Each GPU the library runs
+// on holds its own copy until the process ends.  In host code `scratch` names
+// a host-side stand-in, not device memory: host code reaches the array only
+// through the device address hipGetSymbolAddress returns (scratch_ptr).
+constexpr long long kScratchBytes = 64LL << 20;
+static __device__ __align__(256) float scratch[kScratchBytes / sizeof(float)];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static float *g_scratch_dev[kMaxDevices];  // device address of `scratch` per device (null until first use)
+
+// the current device's copy of `scratch`: looked up once per device, with
+// that device current, on the first static run there
+static int scratch_ptr(float **out)
+{
+    int dev = -1;
+    PA_HIP(hipGetDevice(&dev));
+    PA_CHECK(dev >= 0 && dev < kMaxDevices, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        PA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        PA_CHECK(p != nullptr, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_dev[dev] = (float *)p;
+    }
+    *out = g_scratch_dev[dev];
+    return PYSIT_AMD_OK;
+}
+
+// zero n floats (a kernel, not a memset node, so the run can be captured)
+__global__ void k_zero_f32(float *__restrict__ p, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
 }  // namespace pa
 
 extern "C" {
@@ -339,6 +381,38 @@ int pysit_amd_batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int 
 {
     clear_error();
     return batch_run(b, mode, slot_begin, slot_end, direction, seg_base, active_tiles, active_recs, nullptr,
+                     stream);
+}
+
+int pysit_amd_batch_run_static(const pysit_amd_batch *b, int mode, int slot_begin, int slot_end,
+                               int direction, int seg_base, const int *active_tiles,
+                               const int *active_recs, void *stream)
+{
+    clear_error();
+    PA_CHECK(b != nullptr, "null batch");
+    PA_CHECK(!b->ic32 && !b->inj_pre,
+             "the static run takes no workspace: ic32 and inj_pre must be NULL (its FP32 window "
+             "accumulator is the library's scratch array; steps gather their sources themselves)");
+    pysit_amd_batch B = *b;
+    const bool window = mode == 2 && B.dtype == 0 && B.ic_window >= 1 && B.ic && B.dw;
+    if (window) {
+        PA_CHECK(B.total_n >= 0 && B.total_n * (long long)sizeof(float) <= kScratchBytes,
+                 "static run: the FP32 window accumulator of " + std::to_string(B.total_n) +
+                     " nodes exceeds the scratch array (at most " +
+                     std::to_string(kScratchBytes / (long long)sizeof(float)) +
+                     " nodes); use pysit_amd_batch_run with a caller workspace (ic32)");
+        float *p = nullptr;
+        const int rc = scratch_ptr(&p);
+        if (rc) return rc;
+        // the array keeps the last call's contents: the window sums start at 0
+        if (B.total_n > 0) {
+            hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)((B.total_n + 255) / 256)), dim3(256), 0,
+                               (hipStream_t)stream, p, B.total_n);
+            PA_HIP(hipGetLastError());
+        }
+        B.ic32 = p;
+    }
+    return batch_run(&B, mode, slot_begin, slot_end, direction, seg_base, active_tiles, active_recs, nullptr,
                      stream);
 }
 
