@@ -682,6 +682,7 @@ class DeviceBatch:
         K = segment if segment else self.plan_segments(keep_dw)
         self.K = K
         self.ckpts = []
+        self._dw_seg = None
         self.field_store = self._field_store() if store_field else None
         if keep_dw and K >= self.max_nt:
             self._ensure_dw(self.max_nt)
@@ -699,9 +700,18 @@ class DeviceBatch:
                 self._inj_pre = None
                 torch.cuda.empty_cache()
             b = self._batch(self.fstate, False)
+            self._ensure_dw(K)
+            last = (self.max_nt - 1) // K * K
             for j0 in range(0, self.max_nt, K):
                 self.ckpts.append({f: v.clone() for f, v in self.fstate.items()})
-                self._run(b, 0, j0, min(self.max_nt, j0 + K), 1, j0)
+                if j0 == last:
+                    # the last segment stores its dWaveOp as it goes: migrate
+                    # starts there and needs no recompute for it
+                    bs = self._batch(self.fstate, False, dw=self.dw)
+                    self._run(bs, 1, j0, self.max_nt, 1, j0)
+                else:
+                    self._run(b, 0, j0, min(self.max_nt, j0 + K), 1, j0)
+            self._dw_seg = len(self.ckpts) - 1
         else:
             b = self._batch(self.fstate, False)
             b.fstore = self.field_store.data_ptr() if store_field else None
@@ -846,9 +856,13 @@ class DeviceBatch:
             for si in range(len(self.ckpts) - 1, -1, -1):
                 j0 = si * K
                 j1 = min(self.max_nt, j0 + K)
-                for f, v in self.ckpts[si].items():
-                    work[f].copy_(v)
-                self._run(fb, 1, j0, j1, 1, j0, record=False)
+                if getattr(self, "_dw_seg", None) != si:
+                    # recompute the segment's dWaveOp from its checkpoint (the
+                    # store holds another segment's)
+                    for f, v in self.ckpts[si].items():
+                        work[f].copy_(v)
+                    self._run(fb, 1, j0, j1, 1, j0, record=False)
+                    self._dw_seg = si
                 self._run(qb, 2, j0, j1, -1, j0, record=False)
         return self.ic
 

@@ -211,3 +211,43 @@ def truncated_geometry(g, nt):
     """The same shot with only its first ``nt`` steps."""
     from pysit_amd.device import ShotGeometry
     return ShotGeometry(g.shape, g.kpml, g.window, nt, g.src, g.amp[:nt], g.rec, g.nrec, adj=g.adj)
+
+
+def pml_golden_case(tag, precision="double"):
+    """The two cases of tests/golden/pml_golden.npz built with this package's
+    API (tests/golden/make_pml_golden.py builds them with PySIT's): returns
+    (mesh, shots, solver, true C, start C), C as (dof, 1) arrays."""
+    import pysit_amd as pa
+    if tag == "c1":
+        p = pa.PML(0.1, 100)
+        d = pa.RectangularDomain((0.0, 0.9, p, p), (0.0, 0.7, p, p))
+        m = pa.CartesianMesh(d, 91, 71)
+        X, Z = m.mesh_coords()
+        C = (1.0 + 0.5 * np.exp(-((Z - 0.45) / 0.02) ** 2)).reshape(-1, 1)
+        C0 = (1.0 + 0.3 * np.exp(-((Z - 0.45) / 0.02) ** 2)).reshape(-1, 1)
+        shots = pa.equispaced_acquisition(m, pa.RickerWavelet(10.0), sources=1, source_depth=0.05,
+                                          receivers="max", receiver_depth=0.05)
+        tr = (0.0, 1.5)
+    else:
+        p = pa.PML(0.08, 60)
+        d = pa.RectangularDomain((0.0, 0.3, p, p), (0.0, 0.3, p, p), (0.0, 0.3, p, p))
+        m = pa.CartesianMesh(d, 16, 16, 16)
+        X, Y, Z = m.mesh_coords()
+        C = (1.0 + 0.4 * (Z > 0.17) + 0.15 * np.exp(-((X - 0.15) ** 2 + (Y - 0.15) ** 2 + (Z - 0.1) ** 2) / 0.004))
+        C0 = 1.0 + 0.4 * (Z > 0.17)
+        C, C0 = C.reshape(-1, 1), C0.reshape(-1, 1)
+        rec = np.linspace(0.05, 0.25, 4)
+        src = pa.PointSource(m, (0.15, 0.15, 0.04), pa.RickerWavelet(12.0))
+        recs = pa.ReceiverSet(m, [pa.PointReceiver(m, (a, b, 0.04)) for a in rec for b in rec])
+        shots = [pa.Shot(src, recs)]
+        tr = (0.0, 0.25)
+    solver = pa.ConstantDensityAcousticWave(m, spatial_accuracy_order=4, trange=tr, kernel_implementation="hip",
+                                            precision=precision)
+    return m, shots, solver, C, C0
+
+
+def golden_operand(nt, nrec):
+    """The closed-form migrate operand of tests/golden/make_pml_golden.py."""
+    k = np.arange(nt)[:, None]
+    j = np.arange(nrec)[None, :]
+    return np.sin(0.37 * k + 1.3 * j) * np.exp(-k / 800.0)

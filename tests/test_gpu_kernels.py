@@ -131,6 +131,10 @@ def test_batch_checkpointed_equals_store_all(gpu):
     assert len(b2.ckpts) == 9
     ic2 = b2.migrate(res).cpu().numpy()
     assert np.array_equal(ic1, ic2)
+    # the forward stored the last segment's dWaveOp; a second migrate finds
+    # the first segment's in the store and recomputes the last one
+    ic3 = b2.migrate(res).cpu().numpy()
+    assert np.array_equal(ic1, ic3)
 
 
 def test_batch_uneven_shots_2d(gpu):

@@ -206,3 +206,34 @@ def test_interp_zero_fill_closed_forms():
     inside = (tn >= 0) & (tn <= 1.0)
     want = np.stack([2 * tn + 1, -tn], axis=1) * inside[:, None]
     assert np.allclose(out, want, atol=1e-14)
+
+
+@pytest.mark.parametrize("tag", ["c1", "c3d"])
+def test_oracle_pml_composition_matches_reference_python(tag):
+    """With the PML on, the oracle's forward loop (traces, dWaveOp with the
+    mesh profile and, in 3D, the psi term), its migrate with the fold, and
+    the TemporalLeastSquares objective and gradient built on them (observed
+    data from the oracle in the true model, interpolated onto the start
+    model's time axis) against PySIT's own Python path
+    (tests/golden/pml_golden.npz, written by tests/golden/make_pml_golden.py)."""
+    from tests.problems import golden_operand, oracle_from_solver, pml_golden_case, rel_l2
+    g = np.load(os.path.join(GOLDEN, "pml_golden.npz"))
+    m, shots, solver, C, C0 = pml_golden_case(tag)
+    solver.model_parameters = solver.ModelParameters(m, {"C": C})
+    ts_true = solver.ts()
+    d_true = oracle_from_solver(solver, shots[0]).forward()["simdata"]
+    solver.model_parameters = solver.ModelParameters(m, {"C": C0})
+    assert solver.nsteps == int(g[f"{tag}_nsteps"]) and solver.dt == float(g[f"{tag}_dt"])
+    op = oracle_from_solver(solver, shots[0])
+    fw = op.forward(dwaveop=True)
+    every = int(g[f"{tag}_every"])
+    e = {"traces": rel_l2(fw["simdata"][::every], g[f"{tag}_traces"])}
+    e["dWaveOp"] = max(rel_l2(fw["dWaveOp"][k], g[f"{tag}_dwaveop"][i]) for i, k in enumerate(g[f"{tag}_dw_steps"]))
+    r = golden_operand(*fw["simdata"].shape)
+    e["migrate"] = rel_l2(op.migrate(r), g[f"{tag}_migrate"])
+    res = O.interp_zero_fill(ts_true, d_true, solver.ts()) - fw["simdata"]
+    e["gradient"] = rel_l2(-op.migrate(res), g[f"{tag}_gradient"])
+    f = O.misfit(res, solver.dt)
+    e["objective"] = abs(f - float(g[f"{tag}_objective"])) / float(g[f"{tag}_objective"])
+    print(tag, {k: f"{v:.1e}" for k, v in e.items()})
+    assert all(v < 1e-10 for v in e.values()), e
