@@ -301,6 +301,16 @@ int pysit_amd_plan_stats(const pysit_amd_plan *plan, long long *stats);
 /* Waits for the device, then frees the plan's graphs, cursor and stream. */
 int pysit_amd_plan_destroy(pysit_amd_plan *plan);
 
+/* Sum of squares over time slots of a slot-major store (the dWaveOp store):
+ * out[i] += sum over s = slot_begin, slot_begin + slot_step, ... < slot_end of
+ * store[s * slot_stride + i]^2 in FP64, for i < n (store: dtype 0 f32 or
+ * 1 f64, device).  The per-shot term of the pseudo-Hessian diagonal,
+ * TemporalLeastSquares._pseudo_hessian_diagonal_component_shot
+ * (pysit/objective_functions/temporal_least_squares.py:121-139), before its
+ * crop to the physical grid and its imaging_period * dt factor. */
+int pysit_amd_slot_square_sum(const void *store, int dtype, long long slot_stride, int slot_begin,
+                              int slot_end, int slot_step, long long n, double *out, void *stream);
+
 /* Fold (ModelPerturbationBase.add_padding, pysit/solvers/model_parameter.py:
  * 537-593) of one padded FP64 image onto its unpadded grid: the PML slabs are
  * summed onto the first/last physical plane of each axis (x, then y, then z),

@@ -149,6 +149,23 @@ __global__ void k_inj_pre(const pysit_amd_batch B, int slot_begin, int slot_end,
     }
 }
 
+// out[i] += sum over slots s = s0, s0 + step, ... < s1 of store[s * stride + i]^2
+// (FP64), i < n: one thread per node, the slots summed in order.
+template <typename T>
+__global__ void k_slot_square_sum(const T *__restrict__ store, long long stride, int s0, int s1, int step,
+                                  long long n, double *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int s = s0; s < s1; s += step) {
+        const double v = (double)store[(long long)s * stride + i];
+        acc += v * v;
+    }
+    out[i] += acc;
+}
+
 static int inj_precompute(const pysit_amd_batch &B, int slot_begin, int slot_end, int seg_base, hipStream_t st)
 {
     if (slot_end <= slot_begin || B.n_inj_nodes == 0) return PYSIT_AMD_OK;
@@ -427,6 +444,26 @@ int pysit_amd_plan_run(pysit_amd_plan *plan, const pysit_amd_batch *b, int mode,
     PA_CHECK(dev == plan_device(plan), "the plan belongs to another device");
     return batch_run(b, mode, slot_begin, slot_end, direction, seg_base, active_tiles, active_recs, plan,
                      stream);
+}
+
+int pysit_amd_slot_square_sum(const void *store, int dtype, long long slot_stride, int slot_begin, int slot_end,
+                              int slot_step, long long n, double *out, void *stream)
+{
+    clear_error();
+    PA_CHECK(store && out, "null array argument");
+    PA_CHECK(dtype == 0 || dtype == 1, "dtype must be 0 (f32) or 1 (f64)");
+    PA_CHECK(slot_step >= 1 && slot_begin >= 0 && slot_end >= slot_begin && n >= 0 && slot_stride >= n,
+             "bad slot range, step or stride");
+    if (n == 0 || slot_end == slot_begin) return PYSIT_AMD_OK;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (dtype)
+        hipLaunchKernelGGL(k_slot_square_sum<double>, grid, dim3(256), 0, (hipStream_t)stream,
+                           (const double *)store, slot_stride, slot_begin, slot_end, slot_step, n, out);
+    else
+        hipLaunchKernelGGL(k_slot_square_sum<float>, grid, dim3(256), 0, (hipStream_t)stream,
+                           (const float *)store, slot_stride, slot_begin, slot_end, slot_step, n, out);
+    PA_HIP(hipGetLastError());
+    return PYSIT_AMD_OK;
 }
 
 int pysit_amd_fold(int dim, int nx, int ny, int nz, int lx, int rx, int ly, int ry, int lz, int rz,

@@ -792,12 +792,14 @@ class DeviceBatch:
         return out
 
     # ----------------------------------------------------------------- migrate
-    def migrate(self, residuals, store_field=False, image=True):
+    def migrate(self, residuals, store_field=False, image=True, dw_squares=False):
         """Imaging condition sum_k q_k dWaveOp_k of every shot (padded, FP64,
         concatenated at ``offs``; the caller applies -dt*p when folding).
         ``residuals`` are per-shot (nt, nrec) operands in the caller's shot
         order (numpy or device tensors).  ``store_field`` keeps q_k of every
-        step in ``adjoint_store``; ``image=False`` only propagates q."""
+        step in ``adjoint_store``; ``image=False`` only propagates q.
+        ``dw_squares`` also sums dWaveOp_k^2 over the imaged steps of every
+        shot into ``self.dw_sq`` (padded, FP64, laid out like the images)."""
         if image and not getattr(self, "have_dw", False):
             raise RuntimeError("migrate needs a forward(keep_dw=True) first")
         if self.adj_inj is None:
@@ -817,6 +819,7 @@ class DeviceBatch:
         for f in self.qstate.values():
             f.zero_()
         self.ic = torch.zeros(self.total_n, dtype=torch.float64, device=self.dev)
+        self.dw_sq = torch.zeros(self.total_n, dtype=torch.float64, device=self.dev) if dw_squares else None
         ic32 = torch.zeros(self.total_n, dtype=torch.float32, device=self.dev) if self.ic_window else None
         K = getattr(self, "K", self.max_nt)
         if not hasattr(self, "ckpts"):
@@ -847,6 +850,7 @@ class DeviceBatch:
             self._run(qb, 2, 0, self.max_nt, -1, 0, record=False)
         elif not self.ckpts:
             qb.dw = self.dw.data_ptr()
+            self._dw_squares(0, self.max_nt, 0)
             self._run(qb, 2, 0, self.max_nt, -1, 0, record=False)
         else:
             self._ensure_dw(K)
@@ -863,8 +867,34 @@ class DeviceBatch:
                         work[f].copy_(v)
                     self._run(fb, 1, j0, j1, 1, j0, record=False)
                     self._dw_seg = si
+                self._dw_squares(j0, j1, j0)
                 self._run(qb, 2, j0, j1, -1, j0, record=False)
         return self.ic
+
+    def _dw_squares(self, j0, j1, seg_base):
+        """dw_sq += dWaveOp^2 of the imaged slots in [j0, j1) that the store
+        holds from seg_base on (no-op unless migrate asked for it)."""
+        if self.dw_sq is None:
+            return
+        p = self.imaging_period
+        item = self.dtype.itemsize
+        for pos in range(self.n_shots):
+            ks = self.kshift[pos]
+            first = max(j0, ks)
+            s0 = first + (-(first - ks)) % p  # the first imaged slot at or after `first`
+            if s0 >= j1:
+                continue
+            off = self.offs[pos]
+            N.call("pysit_amd_slot_square_sum", ctypes.c_void_p(self.dw.data_ptr() + off * item),
+                   1 if self.dtype == np.float64 else 0, self.total_n, s0 - seg_base, j1 - seg_base, p,
+                   self.snpts[pos], ctypes.c_void_p(self.dw_sq.data_ptr() + off * 8), _stream())
+
+    def dw_squares_unpadded(self, si_caller, pads):
+        """Sum over the imaged steps of dWaveOp^2 of one shot, cropped to the
+        physical grid (device FP64, the unpadded grid's shape)."""
+        v = self._shot_view(self.dw_sq, self.order_idx.index(si_caller))
+        sl = tuple(slice(l, n - r) for (l, r), n in zip(pads, v.shape))
+        return v[sl]
 
     def _inj_pre_buffer(self, slots, n_nodes):
         """Device buffer for the precomputed injection of ``slots`` slots

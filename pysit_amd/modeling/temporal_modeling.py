@@ -247,11 +247,13 @@ class TemporalModeling(object):
         self.last_batch = None
         return traces
 
-    def gradient_batch(self, shots, m0, operands_fn, imaging_period=1, device=False):
+    def gradient_batch(self, shots, m0, operands_fn, imaging_period=1, device=False, dw_squares=False):
         """Forward all shots, form each shot's operand with ``operands_fn(i,
         simdata)``, migrate them together and return (sum of migrate_shot
         results, list of operands).  With ``device`` the simdata handed to
-        ``operands_fn`` and the operands stay device tensors."""
+        ``operands_fn`` and the operands stay device tensors.  With
+        ``dw_squares`` the batch also sums every shot's dWaveOp^2 over the
+        imaged steps (``last_batch.dw_squares_unpadded``)."""
         solver = self.solver
         solver.model_parameters = m0
         b = self.batch(shots, imaging_period, reuse=True)
@@ -260,7 +262,7 @@ class TemporalModeling(object):
             ops = [operands_fn(i, t) for i, t in enumerate(traces)]
         else:
             ops = [np.asarray(operands_fn(i, t.cpu().numpy()), dtype=np.float64) for i, t in enumerate(traces)]
-        b.migrate(ops)
+        b.migrate(ops, dw_squares=dw_squares)
         g = self._fold_images(b, -solver.dt * imaging_period)
         self.last_batch = b
         return g, ops

@@ -162,6 +162,8 @@ class TemporalLeastSquares(object):
         grad = m0.perturbation()
         r2 = 0.0
         import torch
+        want_h = "pseudo_hess_diag" in aux_info and aux_info["pseudo_hess_diag"][0]
+        hdiag = None
         for group in self._groups(shots):
             ts_holder = {}
 
@@ -169,9 +171,18 @@ class TemporalLeastSquares(object):
                 ts = ts_holder.setdefault("ts", self.solver.ts())
                 return self.observed(group[i], ts, sim.device) - sim
 
-            g, rs = self.modeling_tools.gradient_batch(group, m0, residual, self.imaging_period, device=True)
+            g, rs = self.modeling_tools.gradient_batch(group, m0, residual, self.imaging_period, device=True,
+                                                       dw_squares=want_h)
             grad -= g
             r2 += float(sum(torch.sum(r * r) for r in rs))
+            if want_h:
+                # Shin (2001) illumination: sum over shots and imaged steps of
+                # the physical part of dWaveOp^2 (temporal_least_squares.py:121-139)
+                b = self.modeling_tools.last_batch
+                pads = self.solver.mesh.pad_widths()
+                for i in range(len(group)):
+                    h = b.dw_squares_unpadded(i, pads)
+                    hdiag = h.clone() if hdiag is None else hdiag + h
         if self.parallel_wrap_shot.use_parallel:
             r2 = float(self._allreduce(np.array(r2)))
             grad = m0.perturbation(data=self._allreduce(grad.asarray()))
@@ -180,8 +191,11 @@ class TemporalLeastSquares(object):
             aux_info["residual_norm"] = (True, np.sqrt(r2))
         if "objective_value" in aux_info and aux_info["objective_value"][0]:
             aux_info["objective_value"] = (True, 0.5 * r2)
-        if "pseudo_hess_diag" in aux_info and aux_info["pseudo_hess_diag"][0]:
-            raise NotImplementedError("the pseudo-Hessian diagonal is not computed by the device path")
+        if want_h:
+            h = hdiag.cpu().numpy().reshape(-1, 1) * self.imaging_period
+            if self.parallel_wrap_shot.use_parallel:
+                h = self._allreduce(h)
+            aux_info["pseudo_hess_diag"] = (True, h * self.solver.dt)
         return grad
 
     def apply_hessian(self, shots, m0, m1, hessian_mode="approximate", levenberg_mu=0.0, *args, **kwargs):
