@@ -30,7 +30,11 @@ Cases (closed-form inputs, no random generator in the fixture's recipe):
   40 % and 90 % of the steps, migrate_shot of the closed-form operand
   r[k, j] = sin(0.37 k + 1.3 j) exp(-k / 800), and TemporalLeastSquares'
   objective and gradient at the start model for data generated in the true
-  model.
+  model; then, at the start model, linear_forward_model (Born) traces and
+  apply_hessian for the closed-form perturbation
+  m1 = 0.02 sin(7x + 1) cos(5z), and the objective history and final model
+  of three LBFGS iterations (memory 3, backtracking line search,
+  optimization.py:312-501, lbfgs.py:39-124).
 * ``c3d_*`` -- a 3D layered block (x, y, z [0, 0.3], 16^3 nodes, PML(0.08,
   60) on all sides, order 4, T = 0.25, one 12 Hz Ricker, a 4 x 4 receiver
   patch), the same quantities: traces every 2nd step, dWaveOp at 70 % of the
@@ -85,7 +89,7 @@ def operand(nt, nrec):
     return np.sin(0.37 * k + 1.3 * j) * np.exp(-k / 800.0)
 
 
-def run_case(ps, m, shots, solver, C, C0, every, dw_steps):
+def run_case(ps, m, shots, solver, C, C0, every, dw_steps, second_order=False):
     import copy
     out = {}
     true_model = solver.ModelParameters(m, {"C": C})
@@ -107,7 +111,32 @@ def run_case(ps, m, shots, solver, C, C0, every, dw_steps):
     out["nsteps"] = np.asarray(solver.nsteps)
     out["dt"] = np.asarray(solver.dt)
     out["every"] = np.asarray(every)
+    if second_order:
+        # Born (linear_forward_model) and the Gauss-Newton Hessian applied to
+        # a closed-form perturbation, then three LBFGS iterations (memory 3,
+        # backtracking) from the start model
+        m1 = m0.perturbation(data=born_direction(m))
+        lin = tools.linear_forward_model(shots[0], m0, m1, return_parameters=["simdata"])
+        out["born"] = np.asarray(lin["simdata"])[::every].copy()
+        # apply_hessian('approximate') as the reference composes it
+        # (temporal_least_squares.py:214-222): its own call of migrate_shot
+        # omits the required imaging_period argument (temporal_modeling.py:171)
+        # and raises TypeError, so the composition is spelled out with
+        # imaging_period 1
+        dw0 = tools.forward_model(obs[0], m0, return_parameters=["dWaveOp"])["dWaveOp"]
+        d1 = tools.linear_forward_model(obs[0], m0, m1, return_parameters=["simdata"], dWaveOp0=dw0)["simdata"]
+        out["hessian"] = np.asarray(tools.migrate_shot(obs[0], m0, d1, 1, dWaveOp=dw0).data).reshape(-1).copy()
+        invalg = ps.LBFGS(obj, memory_length=3)
+        res = invalg(obs, m0, 3, line_search="backtrack", status_configuration={"objective_frequency": 1})
+        out["lbfgs_objective"] = np.asarray([v for _, v in sorted(invalg.objective_history.items())])
+        out["lbfgs_C"] = np.asarray(res.C).reshape(-1).copy()
     return out
+
+
+def born_direction(m):
+    """A smooth closed-form model perturbation over the whole physical grid."""
+    X, Z = m.mesh_coords()
+    return (0.02 * np.sin(7.0 * X + 1.0) * np.cos(5.0 * Z)).reshape(-1, 1)
 
 
 def c1(ps):
@@ -121,7 +150,7 @@ def c1(ps):
                                       receivers="max", receiver_depth=0.05)
     solver = ps.ConstantDensityAcousticWave(m, spatial_accuracy_order=4, trange=(0.0, 1.5),
                                             kernel_implementation="cpp")
-    return run_case(ps, m, shots, solver, C, C0, 4, [0.4, 0.9])
+    return run_case(ps, m, shots, solver, C, C0, 4, [0.4, 0.9], second_order=True)
 
 
 def block3d(ps):
