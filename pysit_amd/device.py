@@ -146,6 +146,37 @@ def _as_entries(e, dim):
     return pos, np.asarray(chan, dtype=np.int64).reshape(-1), np.asarray(w, dtype=np.float64).reshape(-1)
 
 
+def checkpoint_levels(nt, per_slot, ckpt, budget, max_levels=4, state=None):
+    """Segment lengths [K_1, ..., K_L] of a nested checkpoint schedule for a
+    run of ``nt`` slots, or None if none fits ``budget`` bytes.
+
+    Level 1 keeps a state checkpoint every K_1 slots of the forward pass;
+    level l > 1 replays one level l-1 segment from its checkpoint and keeps a
+    checkpoint every K_l slots of it; the last level replays a K_L-slot
+    segment storing its dWaveOp (per_slot bytes per slot), which the adjoint
+    then consumes.  Memory: K_L per_slot plus, per level, the checkpoints of
+    one segment of the level above, plus one working state (ckpt bytes each).
+    Each level below the first costs one more forward sweep, so the fewest
+    levels that fit win, and among those the longest stored segment.  The
+    intermediate lengths split the run evenly: K_l ~ nt / r^l with
+    r^L = nt / K_L.  ``state`` is the working state's size (default: a
+    checkpoint's)."""
+    state = ckpt if state is None else state
+    for L in range(1, max_levels + 1):
+        for KL in range(nt, 0, -1):
+            r = (nt / KL) ** (1.0 / L)
+            Ks = [max(KL, -(-nt // int(round(r ** l))) if r ** l >= 1 else nt) for l in range(1, L)] + [KL]
+            Ks = [min(nt, k) for k in Ks]
+            if any(Ks[i] < Ks[i + 1] for i in range(len(Ks) - 1)):
+                continue
+            n = [-(-nt // Ks[0])] + [-(-Ks[i - 1] // Ks[i]) for i in range(1, L)]
+            if KL * per_slot + sum(n) * ckpt + state <= budget:
+                return Ks
+            if sum(n) * ckpt + state > budget and KL == 1:
+                break
+    return None
+
+
 GRAPH_MAX_NODES = 1 << 20
 # FP32 imaging window (slots summed in FP32 before each FP64 flush) and the
 # default graph chunk, a multiple of it so that graph chunks hold whole windows
@@ -565,6 +596,45 @@ class DeviceBatch:
     def state_bytes(self):
         return (len(self._fields()) * 2 + 1) * self.total_n * self.dtype.itemsize
 
+    def ckpt_bytes(self):
+        """A state checkpoint: the current time level of every field and du."""
+        return (len(self._fields()) + 1) * self.total_n * self.dtype.itemsize
+
+    def _levels_at(self, slot):
+        """Per shot, the buffer that holds the current time level of a
+        forward run about to take step ``slot`` (the local step's parity; a
+        shot that has not started is at rest in both)."""
+        return [max(0, slot - ks) & 1 for ks in self.kshift]
+
+    def _ckpt_take(self, state, slot):
+        """Checkpoint of a forward state before step ``slot``.  A step reads
+        only the current level of each field (and du) and rewrites the other
+        level wherever it can be nonzero, so the current level is the whole
+        state."""
+        cur = self._levels_at(slot)
+        out = {"du": state["du"].clone()}
+        for f in self._fields():
+            if len(set(cur)) == 1:
+                out[f] = state[f][cur[0]].clone()
+            else:
+                t = torch.empty(self.total_n, dtype=self.tdtype, device=self.dev)
+                for pos, c in enumerate(cur):
+                    a, n = self.offs[pos], self.snpts[pos]
+                    t[a:a + n].copy_(state[f][c][a:a + n])
+                out[f] = t
+        return out
+
+    def _ckpt_restore(self, ckpt, state, slot):
+        cur = self._levels_at(slot)
+        state["du"].copy_(ckpt["du"])
+        for f in self._fields():
+            if len(set(cur)) == 1:
+                state[f][cur[0]].copy_(ckpt[f])
+            else:
+                for pos, c in enumerate(cur):
+                    a, n = self.offs[pos], self.snpts[pos]
+                    state[f][c][a:a + n].copy_(ckpt[f][a:a + n])
+
     def _batch(self, state, adjoint, ic=None, dw=None):
         b = N.Batch()
         b.dtype = 1 if self.dtype == np.float64 else 0
@@ -639,24 +709,20 @@ class DeviceBatch:
         item = self.dtype.itemsize
         n_inj = int(self.adj_inj[2].numel()) - 1 if self.adj_inj is not None else 0
         per_slot = (self.total_n + n_inj) * item
-        ckpt = self.state_bytes()
+        ckpt = self.ckpt_bytes()
         # what migrate allocates beside the store: the adjoint state (unless
-        # held already) and the FP64 and FP32 imaging accumulators
-        budget -= (0 if getattr(self, "qstate", None) is not None else ckpt) + 12 * self.total_n
+        # held already), the working state of the replays and the FP64 and
+        # FP32 imaging accumulators
+        state = self.state_bytes()
+        budget -= (0 if getattr(self, "qstate", None) is not None else state) + 12 * self.total_n
+        self.levels = [self.max_nt]
         if self.max_nt * per_slot <= budget:
             return self.max_nt
-        # nseg segments: nseg checkpoints, one working state, a K-slot store
-        best = None
-        for nseg in range(2, self.max_nt + 1):
-            K = -(-self.max_nt // nseg)
-            if K * per_slot + (nseg + 1) * ckpt <= budget:
-                best = K
-                break
-            if (nseg + 1) * ckpt > budget:
-                break
-        if best is None:
+        levels = checkpoint_levels(self.max_nt, per_slot, ckpt, budget, state=state)
+        if levels is None:
             raise MemoryError("memory budget too small for the checkpoint schedule")
-        return best
+        self.levels = levels
+        return levels[0]
 
     def _ensure_dw(self, K):
         if self.dw is None or self.dw_slots < K:
@@ -679,7 +745,11 @@ class DeviceBatch:
             t.zero_()
         for f in self.fstate.values():
             f.zero_()
-        K = segment if segment else self.plan_segments(keep_dw)
+        if segment:
+            K = int(segment)
+            self.levels = [K]
+        else:
+            K = self.plan_segments(keep_dw)
         self.K = K
         self.ckpts = []
         self._dw_seg = None
@@ -700,10 +770,11 @@ class DeviceBatch:
                 self._inj_pre = None
                 torch.cuda.empty_cache()
             b = self._batch(self.fstate, False)
-            self._ensure_dw(K)
-            last = (self.max_nt - 1) // K * K
+            one_level = len(getattr(self, "levels", [K])) == 1
+            self._ensure_dw(self.levels[-1] if not one_level else K)
+            last = (self.max_nt - 1) // K * K if one_level else -1
             for j0 in range(0, self.max_nt, K):
-                self.ckpts.append({f: v.clone() for f, v in self.fstate.items()})
+                self.ckpts.append(self._ckpt_take(self.fstate, j0))
                 if j0 == last:
                     # the last segment stores its dWaveOp as it goes: migrate
                     # starts there and needs no recompute for it
@@ -711,7 +782,8 @@ class DeviceBatch:
                     self._run(bs, 1, j0, self.max_nt, 1, j0)
                 else:
                     self._run(b, 0, j0, min(self.max_nt, j0 + K), 1, j0)
-            self._dw_seg = len(self.ckpts) - 1
+            if one_level:
+                self._dw_seg = (last, self.max_nt)
         else:
             b = self._batch(self.fstate, False)
             b.fstore = self.field_store.data_ptr() if store_field else None
@@ -853,23 +925,41 @@ class DeviceBatch:
             self._dw_squares(0, self.max_nt, 0)
             self._run(qb, 2, 0, self.max_nt, -1, 0, record=False)
         else:
-            self._ensure_dw(K)
+            levels = getattr(self, "levels", [K])
+            self._ensure_dw(levels[-1])
             qb.dw = self.dw.data_ptr()
             work = self._new_state()
             fb = self._batch(work, False, dw=self.dw)
+            wb = self._batch(work, False)
             for si in range(len(self.ckpts) - 1, -1, -1):
                 j0 = si * K
-                j1 = min(self.max_nt, j0 + K)
-                if getattr(self, "_dw_seg", None) != si:
-                    # recompute the segment's dWaveOp from its checkpoint (the
-                    # store holds another segment's)
-                    for f, v in self.ckpts[si].items():
-                        work[f].copy_(v)
-                    self._run(fb, 1, j0, j1, 1, j0, record=False)
-                    self._dw_seg = si
-                self._dw_squares(j0, j1, j0)
-                self._run(qb, 2, j0, j1, -1, j0, record=False)
+                self._reverse(j0, min(self.max_nt, j0 + K), self.ckpts[si], 1, levels, work, fb, wb, qb)
         return self.ic
+
+    def _reverse(self, j0, j1, ckpt, lvl, levels, work, fb, wb, qb):
+        """The adjoint over slots [j0, j1) from the state checkpoint taken at
+        j0: at the last level recompute the segment's dWaveOp into the store
+        (unless it holds it) and run the adjoint over it; above, replay the
+        segment keeping a checkpoint every levels[lvl] slots and reverse
+        those sub-segments last to first."""
+        if lvl == len(levels):
+            if getattr(self, "_dw_seg", None) != (j0, j1):
+                self._ckpt_restore(ckpt, work, j0)
+                self._run(fb, 1, j0, j1, 1, j0, record=False)
+                self._dw_seg = (j0, j1)
+            self._dw_squares(j0, j1, j0)
+            self._run(qb, 2, j0, j1, -1, j0, record=False)
+            return
+        Kn = levels[lvl]
+        self._ckpt_restore(ckpt, work, j0)
+        subs = []
+        for s0 in range(j0, j1, Kn):
+            subs.append((s0, min(j1, s0 + Kn), self._ckpt_take(work, s0)))
+            if s0 + Kn < j1:
+                self._run(wb, 0, s0, s0 + Kn, 1, s0, record=False)
+        for s0, s1, c in reversed(subs):
+            self._reverse(s0, s1, c, lvl + 1, levels, work, fb, wb, qb)
+            c.clear()
 
     def _dw_squares(self, j0, j1, seg_base):
         """dw_sq += dWaveOp^2 of the imaged slots in [j0, j1) that the store

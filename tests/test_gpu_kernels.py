@@ -137,6 +137,47 @@ def test_batch_checkpointed_equals_store_all(gpu):
     assert np.array_equal(ic1, ic3)
 
 
+@pytest.mark.parametrize("budget_kib", [2150, 1920])
+def test_nested_checkpoint_levels_equal_store_all(gpu, budget_kib):
+    # memory budgets below what one level of checkpoints needs (240 steps of
+    # a 50 x 70 FP64 shot): the schedule nests two or three levels (replays
+    # with sub-checkpoints) and gives the same bits as storing every step
+    p = Synthetic((50, 70), ((5, 5), (4, 6)), (0.01, 0.01), 240, 8, seed=11)
+    rng = np.random.default_rng(1)
+    res = [rng.standard_normal((p.nt, p.n_rec))]
+    b1 = _batch([p], np.float64)
+    b1.forward(keep_dw=True)
+    ic1 = b1.migrate(res).cpu().numpy()
+    b2 = _batch([p], np.float64, memory_budget=budget_kib * 1024)
+    b2.forward(keep_dw=True)
+    print("checkpoint levels", b2.levels, "top checkpoints", len(b2.ckpts))
+    assert len(b2.levels) >= 2
+    ic2 = b2.migrate(res).cpu().numpy()
+    assert np.array_equal(ic1, ic2)
+
+
+def test_batch_uneven_shots_checkpointed_equal_store_all(gpu):
+    # uneven shots sit at different step parities at a checkpoint slot: each
+    # shot's current level is saved and restored
+    base = Synthetic((80, 60), ((6, 6), (5, 5)), (0.01, 0.01), 100, 8, seed=3)
+    probs = []
+    for s, (w0, w1, nt) in enumerate([(0, 50, 100), (20, 67, 73), (5, 40, 88)]):
+        window = [(6 - w0, w0, w1 - 1), (5, 0, base.model.shape[1] - 1)]
+        probs.append(Synthetic((w1 - w0 + 12, 60), ((6, 6), (5, 5)), (0.01, 0.01), nt, 8, seed=20 + s,
+                               window=window, model=base.model, dt=base.dt))
+    from pysit_amd.device import DeviceBatch
+    rng = np.random.default_rng(9)
+    res = [rng.standard_normal((p.nt, p.n_rec)) for p in probs]
+    out = []
+    for seg in (None, 7):
+        b = DeviceBatch([p.geometry() for p in probs], base.model, base.deltas, base.dt, 8, dtype=np.float64)
+        tr = [t.cpu().numpy().copy() for t in b.forward(keep_dw=True, segment=seg)]
+        b.migrate(res)
+        out.append((tr, b.ic.cpu().numpy().copy()))
+    assert all(np.array_equal(a, c) for a, c in zip(out[0][0], out[1][0]))
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 def test_batch_uneven_shots_2d(gpu):
     base = Synthetic((80, 60), ((6, 6), (5, 5)), (0.01, 0.01), 100, 8, seed=3)
     probs = []
