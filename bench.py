@@ -161,6 +161,40 @@ def cpu_baseline(shape, pml_pairs, dt, deltas, seconds):
                        f"OpenMP step kernels of orders 4 and 6 (2D_4.h, 2D_6.h) on one shot over all threads")}
 
 
+def cpu_gradient(solver, shot, seconds):
+    """One shot's FP64 gradient on the host: the oracle's loops (its C
+    restatement of the reference's step and of forward_model / migrate_shot)
+    built with the reference's flags (-O3 -fopenmp -ffast-math), node loops
+    over all host threads.  The oracle's migrate replays the forward pass to
+    regenerate dWaveOp, so one gradient is four propagations (residual
+    forward, checkpoint pass, dWaveOp recompute, adjoint); the rate counts the
+    metric's two.  The full step count when it fits ``seconds``, else the
+    first steps of the shot (stated in ``sample``)."""
+    from tests.problems import oracle_from_solver
+    from oracle import oracle as O
+    threads = min(len(os.sched_getaffinity(0)), 16)
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    op = oracle_from_solver(solver, shot)
+    n = op.npts
+    full = op.nsteps
+    op.nsteps = min(full, 60)
+    t0 = time.perf_counter()
+    op.forward(fast=True)
+    per_step = (time.perf_counter() - t0) / op.nsteps
+    op.nsteps = int(max(60, min(full, seconds / (4.0 * per_step))))
+    op.amp = op.amp[:op.nsteps]
+    t0 = time.perf_counter()
+    sim = op.forward(fast=True)["simdata"]
+    op.migrate(0.1 * sim, fold=True, fast=True)
+    t = time.perf_counter() - t0
+    return {"value": 2.0 * n * op.nsteps / t / 1e9, "unit": "Gpt-updates/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
+            "kind": "port", "seconds": t,
+            "sample": f"one shot's FP64 gradient on the {'x'.join(map(str, op.shape))} padded grid, "
+                      f"{op.nsteps} of its {full} steps, order 8, the oracle's forward + migrate loops "
+                      f"(-O3 -fopenmp -ffast-math, node loops over {os.environ['OMP_NUM_THREADS']} threads; "
+                      f"four propagations, the rate counts two)"}
+
+
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                             "traffic_k_step2d_layered2d.json")
 
@@ -679,6 +713,7 @@ def main():
     if not (args.no_cpu or args.profile_run):
         kp = solver.kernel_pml()
         result["cpu_baseline"] = cpu_baseline(shape, kp, dt, mesh.deltas, args.cpu_seconds)
+        result["cpu_baseline"]["gradient"] = cpu_gradient(solver, shots[len(shots) // 2], args.cpu_seconds * 4)
     print(json.dumps(result), flush=True)
 
 
