@@ -154,6 +154,24 @@ __device__ __forceinline__ double bcast3(double v, int l)
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// The increment du: adjoint steps may access it non-temporally
+// (PA3D_DU_NT_ADJ), as the 2D adjoint does; forward steps keep it cached.
+#ifndef PA3D_DU_NT_ADJ
+#define PA3D_DU_NT_ADJ 1
+#endif
+template <typename T, int MODE>
+__device__ __forceinline__ T du_ld3(Rsrc r, int voff)
+{
+    if constexpr (adj3(MODE) && PA3D_DU_NT_ADJ) return buf_ld_nt<T>(r, voff, 0);
+    else return buf_ld<T>(r, voff, 0);
+}
+template <typename T, int MODE>
+__device__ __forceinline__ void du_st3(Rsrc r, int voff, T v)
+{
+    if constexpr (adj3(MODE) && PA3D_DU_NT_ADJ) buf_st_nt(r, voff, 0, v);
+    else buf_st(r, voff, 0, v);
+}
+
 // byte offset `o` plus plane offset `p`, kept out of range when o is
 __device__ __forceinline__ int at(int o, int p) { return o == kOOB ? kOOB : o + p; }
 
@@ -193,7 +211,7 @@ __device__ __forceinline__ void march3_int(const Ctx3D<T, R> &c, const Step3DPar
         const int r = (r_), xp = (x0 + r) * pl;                                                     \
         uz[r] = buf_ld<T>(c.u, at(cho, xp), 0);                                                     \
         yh[r] = buf_ld<T>(c.u, at(hro, xp), 0);                                                     \
-        um[r] = buf_ld<T>(INC ? c.du : c.uo, at(ov, xp), 0);                                        \
+        um[r] = INC ? du_ld3<T, MODE>(c.du, at(ov, xp)) : buf_ld<T>(c.uo, at(ov, xp), 0);           \
         const int mp = clampi(x0 + r - c.mx_off, c.mx_lo, c.mx_hi) * c.mplane;                      \
         cv[r] = buf_ld<T>(c.model, mv, mp);                                                         \
         fv[r] = c.finj[r * c.fstride];                                                              \
@@ -232,7 +250,7 @@ __device__ __forceinline__ void march3_int(const Ctx3D<T, R> &c, const Step3DPar
             // increment form: du_{k+1} = du_k + dt^2 S, u_{k+1} = u_k + du_{k+1}
             const T dn = fma(P.tc.dt2, S, um[j]);
             buf_st(c.uo, at(ov, xp), 0, v + dn);
-            buf_st(c.du, at(ov, xp), 0, dn);
+            du_st3<T, MODE>(c.du, at(ov, xp), dn);
         } else {
             buf_st(c.uo, at(ov, xp), 0, fma(P.tc.dt2, S, T(2) * v - um[j]));
         }
@@ -286,7 +304,7 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
         pyh[r] = buf_ld<T>(c.py, (xb || c.bph) ? oh : kOOB, 0);                                     \
         pzc[r] = buf_ld<T>(c.pz, (xb || c.bp) ? oc : kOOB, 0);                                      \
         pzz[r] = buf_ld<T>(c.pz, (xb || c.bpz) ? oz : kOOB, 0);                                     \
-        um[r] = buf_ld<T>(INC ? c.du : c.uo, oo, 0);                                                \
+        um[r] = INC ? du_ld3<T, MODE>(c.du, oo) : buf_ld<T>(c.uo, oo, 0);                           \
         const int mp = clampi(xs - c.mx_off, c.mx_lo, c.mx_hi) * c.mplane;                         \
         cv[r] = buf_ld<T>(c.model, mv, mp);                                                         \
         T f_ = c.has_rhs ? buf_ld<T>(c.rhs, oo, 0) : T(0);                                          \
@@ -425,7 +443,7 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
             buf_st(c.ic, inx ? at(ovi, xp * (8 / SZ)) : kOOB, c.shb8, icv[j] + (double)uc * (double)dv[j]);
         if constexpr (MODE == 4) buf_st(c.ic, inx ? at(ovi, xp) : kOOB, c.shb, icv[j] + uc * dv[j]);
         buf_st(c.uo, oo, 0, up);
-        if constexpr (INC) buf_st(c.du, oo, 0, dn);
+        if constexpr (INC) du_st3<T, MODE>(c.du, oo, dn);
         buf_st(c.pxo, op, 0, pxn);
         buf_st(c.pyo, op, 0, pyn);
         buf_st(c.pzo, op, 0, pzn);
