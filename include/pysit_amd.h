@@ -224,6 +224,22 @@ typedef struct {
      *      transpose after the change of variables Phi' = -dt (sz - sx) mu).
      * Both forms agree wherever the two profiles do (off the right PML).   */
     int dw_form;
+    /* second-order terms of the full Hessian (apply_hessian 'full',
+     * temporal_least_squares.py:224-239); imaging_period must be 1:
+     *   dw_out: when not NULL, mode 3 also stores the dWaveOp of the Born
+     *     field u1 (linear_forward_model's 'dWaveOp1', temporal_modeling.py:
+     *     521-522) and mode 2 the dWaveOp of the adjoint field q
+     *     (adjoint_model's 'dWaveOpAdj', :340-343: the form above applied to
+     *     the adjoint's own three time levels), both in dw's layout;
+     *   vol_m, vol_dw: when not NULL (both), mode 2 adds the volume source
+     *     vol_m[off + idx] * vol_dw[(slot - seg_base) * total_n + off + idx]
+     *     to rhs_k (_setup_adjoint_rhs, :216-226: operand_model, zero-padded,
+     *     times operand_dWaveOpAdj[k]); vol_m is laid out like the state.
+     * A mode 2 run with any of them sums into ic directly (ic32 must be
+     * NULL).                                                                */
+    void *dw_out;
+    const void *vol_m;
+    const void *vol_dw;
 } pysit_amd_batch;
 
 /* Run slots [slot_begin, slot_end) forward (direction = +1) or

@@ -51,6 +51,7 @@ class Batch(ctypes.Structure):
         ("ic32", _vp), ("ic_window", _i),
         ("inj_pre", _vp), ("inj_node_shot", _vp), ("n_inj_nodes", _i), ("inj_pre_slots", _i),
         ("dw_form", _i),
+        ("dw_out", _vp), ("vol_m", _vp), ("vol_dw", _vp),
     ]
 
 

@@ -316,6 +316,15 @@ static int batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int slo
     PA_CHECK(!B.ic32 || (B.dtype == 0 && B.ic && B.ic_window >= 1),
              "the FP32 window accumulator needs an FP32 batch, ic and ic_window >= 1");
     PA_CHECK(mode == 0 || mode == 2 || B.dw, "modes 1 and 3 need the dWaveOp store");
+    PA_CHECK(!B.vol_m == !B.vol_dw, "the volume source needs both vol_m and vol_dw");
+    if (B.dw_out || B.vol_dw) {
+        PA_CHECK(mode == 2 || (mode == 3 && !B.vol_dw),
+                 "dw_out applies to modes 2 and 3, the volume source to mode 2");
+        PA_CHECK(B.imaging_period == 1, "the second-order terms need imaging_period 1");
+        PA_CHECK(!B.ic32, "an adjoint run with second-order terms sums into ic directly (ic32 must be NULL)");
+        PA_CHECK(B.dw_out != B.dw && (!B.vol_dw || B.vol_dw != B.dw_out),
+                 "dw_out must not alias dw or vol_dw");
+    }
     PA_CHECK(B.n_rec == 0 || (B.recs && B.rec_idx && B.rec_w), "receiver tables missing");
     PA_CHECK(seg_base >= 0 && seg_base <= slot_begin, "seg_base must not exceed slot_begin");
     if (active_tiles || active_recs) {
@@ -411,7 +420,7 @@ int pysit_amd_batch_run_static(const pysit_amd_batch *b, int mode, int slot_begi
              "the static run takes no workspace: ic32 and inj_pre must be NULL (its FP32 window "
              "accumulator is the library's scratch array; steps gather their sources themselves)");
     pysit_amd_batch B = *b;
-    const bool window = mode == 2 && B.dtype == 0 && B.ic_window >= 1 && B.ic && B.dw;
+    const bool window = mode == 2 && B.dtype == 0 && B.ic_window >= 1 && B.ic && B.dw && !B.dw_out && !B.vol_dw;
     if (window) {
         PA_CHECK(B.total_n >= 0 && B.total_n * (long long)sizeof(float) <= kScratchBytes,
                  "static run: the FP32 window accumulator of " + std::to_string(B.total_n) +

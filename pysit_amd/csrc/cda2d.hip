@@ -96,6 +96,7 @@ template <typename T, int R, bool PACKED>
 struct Ctx2D {
     State2D<PACKED> s;
     Rsrc model, m1, dw, fst, ic, rhs;  // dw: read (adjoint, Born) or written (forward)
+    Rsrc dwo, volm, vold;              // second-order modes: dWaveOp out, volume source factors
     bool has_fst, has_dwo, has_rhs, has_inj, img, general_phi;
     bool dw_exact;  // store the kernel-profile dWaveOp (C^2 src) in PML tiles too
     const T *finj;  // this lane's LDS source column (or a zero slot)
@@ -112,6 +113,8 @@ struct Ctx2D {
     int ovd;   // owned node where the dWaveOp store is read or written this launch
     int ovi;   // owned node where the imaging accumulator is updated this launch (FP64)
     int ovf;   // owned node where the field store is written
+    int ovo;   // owned node where dw_out is written (modes 5, 6)
+    int ovv;   // owned node where the volume source is read (mode 6)
     int fstride;  // LDS row stride of the tile's sources: 64, or 0 (all rows read zslot)
     int mv;    // model column (edge padding by clamping)
     int zpv;   // centre node where Phi can be nonzero without a PML row
@@ -219,8 +222,13 @@ __device__ __forceinline__ void z_taps(const Ctx2D<T, R, PACKED> &c, int buf, T 
 
 // Kernel modes: 0 propagate, 1 propagate + store dWaveOp, 2 adjoint with the
 // FP64 imaging accumulator, 3 Born, 4 adjoint with the FP32 window
-// accumulator (flushed into the FP64 one by the host loop).
-constexpr bool is_adjoint(int mode) { return mode == 2 || mode == 4; }
+// accumulator (flushed into the FP64 one by the host loop); the full
+// Hessian's second-order runs: 5 Born that also stores its own dWaveOp
+// (dw_out), 6 adjoint with the FP64 accumulator that also stores its own
+// dWaveOp (dw_out) and adds the volume source vol_m * vol_dw_k.
+constexpr bool is_adjoint(int mode) { return mode == 2 || mode == 4 || mode == 6; }
+constexpr bool is_born(int mode) { return mode == 3 || mode == 5; }
+constexpr bool stores_out(int mode) { return mode == 5 || mode == 6; }
 
 // the FP32 window accumulator's read-modify-write may go non-temporal
 // (measured no different)
@@ -284,15 +292,15 @@ __device__ __forceinline__ void ic_store(const C &c, int o, int xr, typename Acc
 #endif
 template <int MODE> __device__ __forceinline__ void uo_st(Rsrc r, int voff, int soff, float v)
 {
-    if constexpr ((MODE == 2 || MODE == 4) && PA2D_UO_NT_ADJ) buf_st_nt(r, voff, soff, v);
+    if constexpr (is_adjoint(MODE) && PA2D_UO_NT_ADJ) buf_st_nt(r, voff, soff, v);
     else buf_st(r, voff, soff, v);
 }
 template <int MODE> __device__ __forceinline__ void uo_st(Rsrc r, int voff, int soff, double v)
 {
-    if constexpr ((MODE == 2 || MODE == 4) && PA2D_UO_NT_ADJ) buf_st_nt(r, voff, soff, v);
+    if constexpr (is_adjoint(MODE) && PA2D_UO_NT_ADJ) buf_st_nt(r, voff, soff, v);
     else buf_st(r, voff, soff, v);
 }
-template <int MODE> constexpr bool du_nt() { return (MODE == 2 || MODE == 4) ? PA2D_DU_NT_ADJ : PA2D_DU_NT_FWD; }
+template <int MODE> constexpr bool du_nt() { return is_adjoint(MODE) ? PA2D_DU_NT_ADJ : PA2D_DU_NT_FWD; }
 template <typename T, int MODE>
 __device__ __forceinline__ T du_ld(Rsrc r, int voff, int soff)
 {
@@ -350,9 +358,10 @@ __device__ __forceinline__ void march_int(const Ctx2D<T, R, PACKED> &c, const St
     const int rb = c.rowb;
     const int so_u = c.s.so(F_U), so_uo = c.s.so(F_UO), so_du = c.s.so(F_DU);
     // um: du_k (batch, increment form) or u_{k-1} (reference-contract step)
-    T u[NU], uh[TX], um[TX], cv[TX], fv[TX], dv[TX], mv[TX];
+    T u[NU], uh[TX], um[TX], cv[TX], fv[TX], dv[TX], mv[TX], wv[TX];
     typename Acc2D<T, MODE>::type icv[TX];
     const int zv = c.zv, hv = c.hv, ov = c.ov, ovd = c.ovd, ovi = c.ovi, ovf = c.ovf, mvo = c.mv;
+    const int ovo = c.ovo, ovv = c.ovv;
 #define PA_LD_U(s_) u[s_] = buf_ld<T>(c.s.rs(F_U), zv + (x0 - R + (s_)) * rb, so_u)
 #define PA_LD_ROW(r_)                                                                               \
     do {                                                                                            \
@@ -364,7 +373,9 @@ __device__ __forceinline__ void march_int(const Ctx2D<T, R, PACKED> &c, const St
         cv[r] = buf_ld<T>(c.model, mvo, mr);                                                        \
         fv[r] = c.finj[r * c.fstride];                                                              \
         dv[r] = (MODE >= 2) ? dw_ld<T>(c.dw, ovd + xr, c.shb) : T(0);                              \
-        mv[r] = (MODE == 3) ? buf_ld<T>(c.m1, mvo, mr) : T(0);                                      \
+        mv[r] = is_born(MODE) ? buf_ld<T>(c.m1, mvo, mr)                                            \
+              : (MODE == 6 ? buf_ld<T>(c.volm, ovv + xr, c.shb) : T(0));                           \
+        wv[r] = (MODE == 6) ? buf_ld<T>(c.vold, ovv + xr, c.shb) : T(0);                            \
         if (is_adjoint(MODE)) icv[r] = ic_load<T, MODE>(c, ovi, xr);                                \
     } while (0)
 #pragma unroll
@@ -381,7 +392,8 @@ __device__ __forceinline__ void march_int(const Ctx2D<T, R, PACKED> &c, const St
         const int xr = x * rb;
         const T v = u[R + j];
         T f = fv[j];
-        if (MODE == 3) f += mv[j] * (-dv[j]);  // Born source: m1_pad * (-dWaveOp0_k)
+        if (is_born(MODE)) f += mv[j] * (-dv[j]);  // Born source: m1_pad * (-dWaveOp0_k)
+        if (MODE == 6) f += mv[j] * wv[j];         // volume source: m1 * dWaveOpAdj_k
         T zp[R + 1], zm[R + 1];
         z_taps<T, R, PACKED>(c, 2 * (j & 1), v, uh[j], zp, zm);
         const T lap = laplacian<T, R>(P, u + j, v, zp, zm);
@@ -398,6 +410,7 @@ __device__ __forceinline__ void march_int(const Ctx2D<T, R, PACKED> &c, const St
             buf_st(c.s.rs(F_UO), ov + xr, so_uo, fma(P.tc.dt2, S, T(2) * v - um[j]));
         }
         if (MODE == 1) dw_st<T>(c.dw, ovd + xr, c.shb, S);
+        if (stores_out(MODE)) dw_st<T>(c.dwo, ovo + xr, c.shb, S);
         if (c.has_fst) buf_st(c.fst, ovf + xr, c.shb, v);
     }
 #undef PA_LD_U
@@ -419,7 +432,8 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
     // values, not fields: a conditional on two fields would select their
     // addresses and keep the context in memory
     const int zv = c.zv, zpv = c.zpv, hv = c.hv, hpv = c.hpv, ov = c.ov, ovd = c.ovd, ovi = c.ovi, ovf = c.ovf;
-    T u[NU], px[NU], uh[TX], um[TX], cv[TX], fv[TX], dv[TX], mv[TX], pv[TX], ph[TX];
+    const int ovo = c.ovo, ovv = c.ovv;
+    T u[NU], px[NU], uh[TX], um[TX], cv[TX], fv[TX], dv[TX], mv[TX], wv[TX], pv[TX], ph[TX];
     typename Acc2D<T, MODE>::type icv[TX];
     const int mvo = c.mv;
 #define PA_LD_U(s_)                                                                                 \
@@ -447,7 +461,9 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
         f += c.finj[r * c.fstride];                                                                 \
         fv[r] = f;                                                                                  \
         dv[r] = (MODE >= 2) ? dw_ld<T>(c.dw, own ? ovd + xr : kOOB, c.shb) : T(0);                 \
-        mv[r] = (MODE == 3) ? buf_ld<T>(c.m1, mvo, mr) : T(0);                                      \
+        mv[r] = is_born(MODE) ? buf_ld<T>(c.m1, mvo, mr)                                            \
+              : (MODE == 6 ? buf_ld<T>(c.volm, own ? ovv + xr : kOOB, c.shb) : T(0));              \
+        wv[r] = (MODE == 6) ? buf_ld<T>(c.vold, own ? ovv + xr : kOOB, c.shb) : T(0);               \
         if (is_adjoint(MODE)) icv[r] = ic_load<T, MODE>(c, own ? ovi : kOOB, xr);                  \
         pv[r] = buf_ld<T>(c.s.rs(F_PZ), own ? (xp ? zv : zpv) + xr : kOOB, so_pz);                  \
         ph[r] = buf_ld<T>(c.s.rs(F_PZ), own ? (xp ? hv : hpv) + xr : kOOB, so_pz);                  \
@@ -476,7 +492,8 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
         const int ovr = own ? ov + xr : kOOB;
         const T v = u[R + j];
         T f = fv[j];
-        if (MODE == 3) f += mv[j] * (-dv[j]);
+        if (is_born(MODE)) f += mv[j] * (-dv[j]);
+        if (MODE == 6) f += mv[j] * wv[j];
         T zp[R + 1], zm[R + 1], pzp[R + 1], pzm[R + 1];
         z_taps<T, R, PACKED>(c, 2 * (j & 1), v, uh[j], zp, zm);
         z_taps<T, R, PACKED>(c, 2 * (j & 1) + 1, pv[j], ph[j], pzp, pzm);
@@ -563,6 +580,7 @@ __device__ __forceinline__ void march_full(const Ctx2D<T, R, PACKED> &c, const S
         buf_st(c.s.rs(F_PXO), op, c.s.so(F_PXO), pxn);
         buf_st(c.s.rs(F_PZO), op, c.s.so(F_PZO), pzn);
         if (MODE == 1) dw_st<T>(c.dw, own ? ovd + xr : kOOB, c.shb, dw);
+        if (stores_out(MODE)) dw_st<T>(c.dwo, own ? ovo + xr : kOOB, c.shb, dw);
         if (c.has_fst) buf_st(c.fst, own ? ovf + xr : kOOB, c.shb, v);
     }
 #undef PA_LD_U
@@ -651,17 +669,31 @@ __device__ __forceinline__ void tile2d(const pysit_amd_batch &B, const Step2DPar
     const bool store = (MODE == 1) && imaged && B.dw;
     const long long slot_el = (long long)(slot - seg_base) * B.total_n;
     const long long slot_bytes = B.total_n * SZ;
-    const bool dwon = ((c.img || MODE == 3) && B.dw) || store;
+    const bool dwon = ((c.img || is_born(MODE)) && B.dw) || store;
     c.dw = make_rsrc(dwon ? (const void *)((T *)B.dw + slot_el) : vbase, dwon ? slot_bytes : 0);
     c.has_dwo = store;
     c.has_fst = B.fstore != nullptr;
     c.fst = make_rsrc(c.has_fst ? (const void *)((T *)B.fstore + slot_el) : vbase, c.has_fst ? slot_bytes : 0);
     if constexpr (MODE == 4) c.ic = make_rsrc(B.ic32, B.total_n * 4);
-    else c.ic = make_rsrc(MODE == 2 ? (const void *)B.ic : vbase, MODE == 2 ? B.total_n * 8 : 0);
+    else c.ic = make_rsrc((MODE == 2 || MODE == 6) ? (const void *)B.ic : vbase,
+                          (MODE == 2 || MODE == 6) ? B.total_n * 8 : 0);
     c.ovd = dwon ? c.ov : kOOB;
     c.ovi = c.img ? (MODE == 4 ? c.ov : c.ov8) : kOOB;
     c.ovf = c.has_fst ? c.ov : kOOB;
-    c.m1 = make_rsrc(MODE == 3 ? B.born_m1 : vbase, MODE == 3 ? (long long)(S.mx_hi + 1) * S.mnz * SZ : 0);
+    c.m1 = make_rsrc(is_born(MODE) ? B.born_m1 : vbase, is_born(MODE) ? (long long)(S.mx_hi + 1) * S.mnz * SZ : 0);
+    c.ovo = kOOB;
+    c.ovv = kOOB;
+    if constexpr (stores_out(MODE)) {
+        const bool out = B.dw_out != nullptr;
+        c.dwo = make_rsrc(out ? (const void *)((T *)B.dw_out + slot_el) : vbase, out ? slot_bytes : 0);
+        c.ovo = out ? c.ov : kOOB;
+    }
+    if constexpr (MODE == 6) {
+        const bool vol = B.vol_m && B.vol_dw;
+        c.volm = make_rsrc(vol ? B.vol_m : vbase, vol ? slot_bytes : 0);
+        c.vold = make_rsrc(vol ? (const void *)((const T *)B.vol_dw + slot_el) : vbase, vol ? slot_bytes : 0);
+        c.ovv = vol ? c.ov : kOOB;
+    }
 
     // sources of this tile, gathered into LDS (row-major TX x 64): one lane
     // per destination node, each summing its entries in the reference's order
@@ -883,12 +915,16 @@ struct Sink2D final : StepSink {
         : B(B_), P(P_), r(r_), mode(mode_), window(window_) {}
     int step(int slot, int seg_base, int nt, int nr, const int *cursor, hipStream_t st) override
     {
+        const bool second = B.dw_out || B.vol_dw;
         switch (mode) {
         case 0: return launch2d_r<T, 0>(r, B, P, slot, seg_base, nt, nr, st, cursor);
         case 1: return launch2d_r<T, 1>(r, B, P, slot, seg_base, nt, nr, st, cursor);
-        case 3: return launch2d_r<T, 3>(r, B, P, slot, seg_base, nt, nr, st, cursor);
+        case 3:
+            if (second) return launch2d_r<T, 5>(r, B, P, slot, seg_base, nt, nr, st, cursor);
+            return launch2d_r<T, 3>(r, B, P, slot, seg_base, nt, nr, st, cursor);
         default: break;
         }
+        if (second) return launch2d_r<T, 6>(r, B, P, slot, seg_base, nt, 0, st, cursor);
         if (!window) return launch2d_r<T, 2>(r, B, P, slot, seg_base, nt, 0, st, cursor);
         if constexpr (std::is_same<T, float>::value)
             return launch2d_r<T, 4>(r, B, P, slot, seg_base, nt, 0, st, cursor);

@@ -35,8 +35,14 @@ namespace pa {
 
 // Kernel modes: 0 propagate, 1 + dWaveOp store, 2 adjoint with the FP64
 // imaging accumulator, 3 Born, 4 adjoint with the FP32 window accumulator
-// (flushed into the FP64 one by the host loop, as in 2D).
-constexpr bool adj3(int mode) { return mode == 2 || mode == 4; }
+// (flushed into the FP64 one by the host loop, as in 2D); the full Hessian's
+// second-order runs: 5 Born that also stores its own dWaveOp (dw_out), 6
+// adjoint with the FP64 accumulator that also stores its own dWaveOp and adds
+// the volume source vol_m * vol_dw_k.
+constexpr bool adj3(int mode) { return mode == 2 || mode == 4 || mode == 6; }
+constexpr bool born3(int mode) { return mode == 3 || mode == 5; }
+constexpr bool out3(int mode) { return mode == 5 || mode == 6; }
+constexpr bool ic64_3(int mode) { return mode == 2 || mode == 6; }
 template <int MODE> struct Acc3D { typedef double type; };
 template <> struct Acc3D<4> { typedef float type; };
 
@@ -98,11 +104,13 @@ struct Ctx3D {
     // u: u_k; uo: u_{k+1} (increment form) or u_{k-1}, overwritten by u_{k+1}
     // (the reference-contract step); du: the increment du_k = u_k - u_{k-1}
     Rsrc u, uo, du, px, pxo, py, pyo, pz, pzo, ps, pso, model, m1, dw, fst, ic, rhs;
+    Rsrc dwo, volm, vold;    // second-order modes: dWaveOp out, volume source factors
     bool has_fst, has_rhs;
     int nx, ny, nz, lane, w, y;
     int plane;               // bytes per x plane of the shot grid
     int co, cho, hro;        // own row centre, own row z-halo value, y-halo row centre (kOOB if none)
     int ov, ov8, ovd, ovi, ovf;
+    int ovo, ovv;            // owned node where dw_out is written / the volume source read (kOOB if not)
     int mv, mplane, mx_off, mx_lo, mx_hi;
     int shb, shb8;
     int hl;                  // LDS index (in a plane buffer) of this lane's z-halo slot in its own row, or -1
@@ -202,8 +210,8 @@ __device__ __forceinline__ void march3_int(const Ctx3D<T, R> &c, const Step3DPar
     constexpr int SZ = sizeof(T);
     const int pl = c.plane;
     const int co = c.co, cho = c.cho, hro = c.hro, ov = c.ov, mv = c.mv;
-    const int ovd = c.ovd, ovi = c.ovi, ovf = c.ovf;
-    T q[NQ], uz[TX], yh[TX], um[TX], cv[TX], fv[TX], dv[TX], mm[TX];
+    const int ovd = c.ovd, ovi = c.ovi, ovf = c.ovf, ovo = c.ovo, ovv = c.ovv;
+    T q[NQ], uz[TX], yh[TX], um[TX], cv[TX], fv[TX], dv[TX], mm[TX], wv[TX];
     typename Acc3D<MODE>::type icv[TX];
 #define PA_Q(s_) q[s_] = buf_ld<T>(c.u, at(co, (x0 - R + (s_)) * pl), 0)
 #define PA_ROW(r_)                                                                                  \
@@ -216,8 +224,10 @@ __device__ __forceinline__ void march3_int(const Ctx3D<T, R> &c, const Step3DPar
         cv[r] = buf_ld<T>(c.model, mv, mp);                                                         \
         fv[r] = c.finj[r * c.fstride];                                                              \
         dv[r] = (MODE >= 2) ? buf_ld<T>(c.dw, at(ovd, xp), c.shb) : T(0);                           \
-        mm[r] = (MODE == 3) ? buf_ld<T>(c.m1, mv, mp) : T(0);                                       \
-        if constexpr (MODE == 2) icv[r] = buf_ld<double>(c.ic, at(ovi, xp * (8 / SZ)), c.shb8);     \
+        mm[r] = born3(MODE) ? buf_ld<T>(c.m1, mv, mp)                                               \
+              : (MODE == 6 ? buf_ld<T>(c.volm, at(ovv, xp), c.shb) : T(0));                         \
+        wv[r] = (MODE == 6) ? buf_ld<T>(c.vold, at(ovv, xp), c.shb) : T(0);                         \
+        if constexpr (ic64_3(MODE)) icv[r] = buf_ld<double>(c.ic, at(ovi, xp * (8 / SZ)), c.shb8);  \
         if constexpr (MODE == 4) icv[r] = buf_ld<float>(c.ic, at(ovi, xp), c.shb);                    \
     } while (0)
 #pragma unroll
@@ -239,12 +249,13 @@ __device__ __forceinline__ void march3_int(const Ctx3D<T, R> &c, const Step3DPar
         lds_barrier();
         const int x = x0 + j, xp = x * pl;
         T f = fv[j];
-        if (MODE == 3) f += mm[j] * (-dv[j]);
+        if (born3(MODE)) f += mm[j] * (-dv[j]);
+        if (MODE == 6) f += mm[j] * wv[j];
         const T lap = laplacian3<T, R, LZ>(P, q + j, lu, own, v);
         const T c2 = cv[j] * cv[j];
         const T src = f + lap;
         const T S = c2 * src;
-        if constexpr (MODE == 2) buf_st(c.ic, at(ovi, xp * (8 / SZ)), c.shb8, icv[j] + (double)v * (double)dv[j]);
+        if constexpr (ic64_3(MODE)) buf_st(c.ic, at(ovi, xp * (8 / SZ)), c.shb8, icv[j] + (double)v * (double)dv[j]);
         if constexpr (MODE == 4) buf_st(c.ic, at(ovi, xp), c.shb, icv[j] + v * dv[j]);
         if constexpr (INC) {
             // increment form: du_{k+1} = du_k + dt^2 S, u_{k+1} = u_k + du_{k+1}
@@ -255,6 +266,7 @@ __device__ __forceinline__ void march3_int(const Ctx3D<T, R> &c, const Step3DPar
             buf_st(c.uo, at(ov, xp), 0, fma(P.tc.dt2, S, T(2) * v - um[j]));
         }
         if (MODE == 1) buf_st(c.dw, at(ovd, xp), c.shb, S);
+        if (out3(MODE)) buf_st(c.dwo, at(ovo, xp), c.shb, S);
         if (c.has_fst) buf_st(c.fst, at(ovf, xp), c.shb, v);
     }
 #undef PA_Q
@@ -273,10 +285,10 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
     constexpr int SZ = sizeof(T);
     const int pl = c.plane, nx = c.nx;
     const int co = c.co, cho = c.cho, hro = c.hro, ov = c.ov, mv = c.mv;
-    const int ovd = c.ovd, ovi = c.ovi, ovf = c.ovf;
+    const int ovd = c.ovd, ovi = c.ovi, ovf = c.ovf, ovo = c.ovo, ovv = c.ovv;
     T q[NQ], qx[NQ], qs[NQ];
     T uz[TX], yh[TX], sz[TX], sh[TX], pyc[TX], pyh[TX], pzc[TX], pzz[TX];
-    T um[TX], cv[TX], fv[TX], dv[TX], mm[TX];
+    T um[TX], cv[TX], fv[TX], dv[TX], mm[TX], wv[TX];
     typename Acc3D<MODE>::type icv[TX];
 #define PA_Q(s_)                                                                                    \
     do {                                                                                            \
@@ -311,8 +323,10 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
         f_ += c.finj[r * c.fstride];                                                                \
         fv[r] = f_;                                                                                 \
         dv[r] = (MODE >= 2) ? buf_ld<T>(c.dw, in ? at(ovd, xp) : kOOB, c.shb) : T(0);               \
-        mm[r] = (MODE == 3) ? buf_ld<T>(c.m1, mv, mp) : T(0);                                       \
-        if constexpr (MODE == 2) icv[r] = buf_ld<double>(c.ic, in ? at(ovi, xp * (8 / SZ)) : kOOB, c.shb8); \
+        mm[r] = born3(MODE) ? buf_ld<T>(c.m1, mv, mp)                                               \
+              : (MODE == 6 ? buf_ld<T>(c.volm, in ? at(ovv, xp) : kOOB, c.shb) : T(0));            \
+        wv[r] = (MODE == 6) ? buf_ld<T>(c.vold, in ? at(ovv, xp) : kOOB, c.shb) : T(0);             \
+        if constexpr (ic64_3(MODE)) icv[r] = buf_ld<double>(c.ic, in ? at(ovi, xp * (8 / SZ)) : kOOB, c.shb8); \
         if constexpr (MODE == 4) icv[r] = buf_ld<float>(c.ic, in ? at(ovi, xp) : kOOB, c.shb);         \
     } while (0)
 #pragma unroll
@@ -349,7 +363,8 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
         const bool inx = x < nx;
         const int xp = (inx ? x : nx - 1) * pl;
         T f = fv[j];
-        if (MODE == 3) f += mm[j] * (-dv[j]);
+        if (born3(MODE)) f += mm[j] * (-dv[j]);
+        if (MODE == 6) f += mm[j] * wv[j];
         const T lap = laplacian3<T, R, LZ>(P, q + j, lu, own, uc);
         const T c2 = cv[j] * cv[j];
         const T u_m = um[j];
@@ -439,7 +454,7 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
         const int oo = inx ? at(ov, xp) : kOOB;
         // Phi keeps its value (zero) where the node is not a PML node
         const int op = (c.gen || pml || edge) ? oo : kOOB;
-        if constexpr (MODE == 2)
+        if constexpr (ic64_3(MODE))
             buf_st(c.ic, inx ? at(ovi, xp * (8 / SZ)) : kOOB, c.shb8, icv[j] + (double)uc * (double)dv[j]);
         if constexpr (MODE == 4) buf_st(c.ic, inx ? at(ovi, xp) : kOOB, c.shb, icv[j] + uc * dv[j]);
         buf_st(c.uo, oo, 0, up);
@@ -449,6 +464,7 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
         buf_st(c.pzo, op, 0, pzn);
         buf_st(c.pso, oo, 0, psn);
         if (MODE == 1) buf_st(c.dw, inx ? at(ovd, xp) : kOOB, c.shb, dw);
+        if (out3(MODE)) buf_st(c.dwo, inx ? at(ovo, xp) : kOOB, c.shb, dw);
         if (c.has_fst) buf_st(c.fst, inx ? at(ovf, xp) : kOOB, c.shb, uc);
     }
 #undef PA_Q
@@ -540,17 +556,30 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
     const bool store = (MODE == 1) && (k % B.imaging_period == 0) && B.dw;
     const long long slot_el = (long long)(slot - seg_base) * B.total_n;
     const long long slot_bytes = B.total_n * SZ;
-    const bool dwon = ((img || MODE == 3) && B.dw) || store;
+    const bool dwon = ((img || born3(MODE)) && B.dw) || store;
     c.dw = make_rsrc(dwon ? (const void *)((const T *)B.dw + slot_el) : vb, dwon ? slot_bytes : 0);
     c.has_fst = B.fstore != nullptr;
     c.fst = make_rsrc(c.has_fst ? (const void *)((const T *)B.fstore + slot_el) : vb, c.has_fst ? slot_bytes : 0);
     if constexpr (MODE == 4) c.ic = make_rsrc(B.ic32, B.total_n * 4);
-    else c.ic = make_rsrc(MODE == 2 ? (const void *)B.ic : vb, MODE == 2 ? B.total_n * 8 : 0);
-    c.m1 = make_rsrc(MODE == 3 ? B.born_m1 : vb,
-                     MODE == 3 ? (long long)(S.mx_hi + 1) * S.mny * S.mnz * SZ : 0);
+    else c.ic = make_rsrc(ic64_3(MODE) ? (const void *)B.ic : vb, ic64_3(MODE) ? B.total_n * 8 : 0);
+    c.m1 = make_rsrc(born3(MODE) ? B.born_m1 : vb,
+                     born3(MODE) ? (long long)(S.mx_hi + 1) * S.mny * S.mnz * SZ : 0);
     c.ovd = dwon ? c.ov : kOOB;
     c.ovi = img ? (MODE == 4 ? c.ov : c.ov8) : kOOB;
     c.ovf = c.has_fst ? c.ov : kOOB;
+    c.ovo = kOOB;
+    c.ovv = kOOB;
+    if constexpr (out3(MODE)) {
+        const bool out = B.dw_out != nullptr;
+        c.dwo = make_rsrc(out ? (const void *)((T *)B.dw_out + slot_el) : vb, out ? slot_bytes : 0);
+        c.ovo = out ? c.ov : kOOB;
+    }
+    if constexpr (MODE == 6) {
+        const bool vol = B.vol_m && B.vol_dw;
+        c.volm = make_rsrc(vol ? B.vol_m : vb, vol ? slot_bytes : 0);
+        c.vold = make_rsrc(vol ? (const void *)((const T *)B.vol_dw + slot_el) : vb, vol ? slot_bytes : 0);
+        c.ovv = vol ? c.ov : kOOB;
+    }
     // PML profiles: lane y/z values and the tile planes' x profile (lane p:
     // kernel form of plane x0+p, lane TX+p: mesh form), masked buffer loads
     static_assert(2 * TX <= 64, "the x profile of a tile must fit one wavefront");
@@ -711,16 +740,20 @@ struct Sink3D final : StepSink {
         : B(B_), P(P_), r(r_), mode(mode_), window(window_) {}
     int step(int slot, int seg_base, int nt, int nr, const int *cursor, hipStream_t st) override
     {
+        const bool second = B.dw_out || B.vol_dw;
         switch (mode) {
         case 0: return launch3d_r<T, 0>(r, B, P, slot, seg_base, nt, nr, st, cursor);
         case 1: return launch3d_r<T, 1>(r, B, P, slot, seg_base, nt, nr, st, cursor);
         case 2:
+            if (second) return launch3d_r<T, 6>(r, B, P, slot, seg_base, nt, 0, st, cursor);
             if (!window) return launch3d_r<T, 2>(r, B, P, slot, seg_base, nt, 0, st, cursor);
             if constexpr (std::is_same<T, float>::value)
                 return launch3d_r<T, 4>(r, B, P, slot, seg_base, nt, 0, st, cursor);
             set_error("the FP32 window accumulator needs an FP32 batch");
             return PYSIT_AMD_EINVAL;
-        default: return launch3d_r<T, 3>(r, B, P, slot, seg_base, nt, nr, st, cursor);
+        default:
+            if (second) return launch3d_r<T, 5>(r, B, P, slot, seg_base, nt, nr, st, cursor);
+            return launch3d_r<T, 3>(r, B, P, slot, seg_base, nt, nr, st, cursor);
         }
     }
     int flush(hipStream_t st) override { return ic_flush(B, st); }

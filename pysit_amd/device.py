@@ -592,6 +592,8 @@ class DeviceBatch:
         self.dw = None
         self.dw_slots = 0
         self.ic = None
+        self.dw1 = None     # dWaveOp of the Born field (born(store_dw=True))
+        self.dw_adj = None  # dWaveOp of the adjoint field (migrate(store_adjoint_dw=True))
 
     def state_bytes(self):
         return (len(self._fields()) * 2 + 1) * self.total_n * self.dtype.itemsize
@@ -745,6 +747,9 @@ class DeviceBatch:
             t.zero_()
         for f in self.fstate.values():
             f.zero_()
+        # histories derived from the previous forward state
+        self.dw1 = None
+        self.dw_adj = None
         if segment:
             K = int(segment)
             self.levels = [K]
@@ -831,16 +836,49 @@ class DeviceBatch:
         slot = k + self.kshift[pos]
         return self._shot_view(store[slot], pos)
 
-    def shot_dw(self, si_caller, k):
-        """Stored dWaveOp of step k of one shot (store-all histories only)."""
-        if self.ckpts:
+    def shot_dw(self, si_caller, k, store="dw"):
+        """Stored dWaveOp of step k of one shot (store-all histories only):
+        the forward's (``"dw"``), the Born field's (``"dw1"``, from
+        ``born(store_dw=True)``) or the adjoint field's (``"dw_adj"``, from
+        ``migrate(store_adjoint_dw=True)``)."""
+        if store == "dw" and self.ckpts:
             raise RuntimeError("dWaveOp was checkpointed, not stored")
-        return self.shot_field(self.dw, si_caller, k)
+        hist = getattr(self, store, None)
+        if hist is None:
+            raise RuntimeError(f"no {store} history is held")
+        return self.shot_field(hist, si_caller, k)
 
-    def born(self, m1):
+    def padded_operand(self, si_caller, grid):
+        """A flat batch array (dtype of the batch, laid out like the state)
+        holding ``grid`` (the padded grid of one shot) in that shot's region
+        and zeros elsewhere."""
+        pos = self.order_idx.index(si_caller)
+        out = torch.zeros(self.total_n, dtype=self.tdtype, device=self.dev)
+        g = torch.as_tensor(np.ascontiguousarray(np.asarray(grid, dtype=self.dtype).reshape(self.shots[pos].shape)))
+        self._shot_view(out, pos).copy_(g.to(self.dev))
+        return out
+
+    def history_from(self, si_caller, images):
+        """A (max_nt, total_n) device history holding host arrays of one shot
+        (one padded grid per step from step 0, imaging period 1) in that
+        shot's slots and zeros elsewhere."""
+        hist = torch.zeros((self.max_nt, self.total_n), dtype=self.tdtype, device=self.dev)
+        pos = self.order_idx.index(si_caller)
+        s = self.shots[pos]
+        for k, a in enumerate(images):
+            if k >= s.nt:
+                break
+            src = torch.as_tensor(np.ascontiguousarray(np.asarray(a).reshape(s.shape), dtype=self.dtype))
+            self._shot_view(hist[k + self.kshift[pos]], pos).copy_(src.to(self.dev))
+        return hist
+
+    def born(self, m1, store_dw=False, store_field=False):
         """Traces of the Born wavefield u1 driven by -m1 * dWaveOp0_k (mode 3);
         needs the whole background dWaveOp history from forward(keep_dw=True).
-        ``m1`` has the model's shape.  Returns per-shot traces (caller order)."""
+        ``m1`` has the model's shape.  With ``store_dw`` the dWaveOp of u1 of
+        every step is kept in ``self.dw1`` (linear_forward_model's 'dWaveOp1'),
+        with ``store_field`` u1 of every step in ``field_store``.  Returns
+        per-shot traces (caller order)."""
         if not getattr(self, "have_dw", False) or self.ckpts or self.imaging_period != 1:
             raise RuntimeError("born needs forward(keep_dw=True) with every step stored (imaging_period 1)")
         m1 = torch.as_tensor(np.ascontiguousarray(m1, dtype=self.dtype)).to(self.dev)
@@ -854,6 +892,14 @@ class DeviceBatch:
         m1 = self._kernel_layout(m1)
         b.born_m1 = m1.data_ptr()
         self._born_keep = m1
+        if store_dw:
+            # zeroed: slots before an uneven shot starts and the row padding
+            # are never written
+            self.dw1 = None
+            self.dw1 = torch.zeros((self.max_nt, self.total_n), dtype=self.tdtype, device=self.dev)
+            b.dw_out = self.dw1.data_ptr()
+        self.field_store = self._field_store() if store_field else None
+        b.fstore = self.field_store.data_ptr() if store_field else None
         self._run(b, 3, 0, self.max_nt, 1, 0)
         return self.traces_in_caller_order()
 
@@ -864,18 +910,33 @@ class DeviceBatch:
         return out
 
     # ----------------------------------------------------------------- migrate
-    def migrate(self, residuals, store_field=False, image=True, dw_squares=False):
+    def migrate(self, residuals, store_field=False, image=True, dw_squares=False, image_store="dw",
+                store_adjoint_dw=False, volume=None):
         """Imaging condition sum_k q_k dWaveOp_k of every shot (padded, FP64,
         concatenated at ``offs``; the caller applies -dt*p when folding).
         ``residuals`` are per-shot (nt, nrec) operands in the caller's shot
         order (numpy or device tensors).  ``store_field`` keeps q_k of every
         step in ``adjoint_store``; ``image=False`` only propagates q.
         ``dw_squares`` also sums dWaveOp_k^2 over the imaged steps of every
-        shot into ``self.dw_sq`` (padded, FP64, laid out like the images)."""
+        shot into ``self.dw_sq`` (padded, FP64, laid out like the images).
+
+        The full Hessian's terms (every step stored, imaging period 1):
+        ``image_store`` names the history imaged against ("dw", or "dw1" of
+        born(store_dw=True)); ``store_adjoint_dw`` keeps the dWaveOp of q of
+        every step in ``self.dw_adj``; ``volume`` = (vol_m, history) adds the
+        source vol_m * history_k (vol_m: a flat batch array, see
+        padded_operand; history: a (max_nt, total_n) store)."""
+        second = image_store != "dw" or store_adjoint_dw or volume is not None
+        if second and (self.imaging_period != 1 or (image and self.ckpts)):
+            raise RuntimeError("an adjoint with second-order terms needs imaging_period 1 and, when it images, "
+                               "every step of the forward stored")
         if image and not getattr(self, "have_dw", False):
             raise RuntimeError("migrate needs a forward(keep_dw=True) first")
         if self.adj_inj is None:
             raise RuntimeError("the shots carry no adjoint injection entries")
+        img_hist = getattr(self, image_store, None) if image else None
+        if image and img_hist is None:
+            raise RuntimeError(f"no {image_store} history is held")
         res = [None] * self.n_shots
         for pos, orig in enumerate(self.order_idx):
             r = residuals[orig]
@@ -892,7 +953,8 @@ class DeviceBatch:
             f.zero_()
         self.ic = torch.zeros(self.total_n, dtype=torch.float64, device=self.dev)
         self.dw_sq = torch.zeros(self.total_n, dtype=torch.float64, device=self.dev) if dw_squares else None
-        ic32 = torch.zeros(self.total_n, dtype=torch.float32, device=self.dev) if self.ic_window else None
+        ic32 = torch.zeros(self.total_n, dtype=torch.float32, device=self.dev) \
+            if (self.ic_window and not second) else None
         K = getattr(self, "K", self.max_nt)
         if not hasattr(self, "ckpts"):
             self.ckpts = []
@@ -917,11 +979,23 @@ class DeviceBatch:
             qb.fstore = self.adjoint_store.data_ptr()
             if self.ckpts and image:
                 raise ValueError("storing the adjoint field needs seg_base 0 (no checkpoints)")
+        if store_adjoint_dw:
+            self.dw_adj = None
+            self.dw_adj = torch.zeros((self.max_nt, self.total_n), dtype=self.tdtype, device=self.dev)
+            qb.dw_out = self.dw_adj.data_ptr()
+        if volume is not None:
+            vm, vd = volume
+            if tuple(vm.shape) != (self.total_n,) or tuple(vd.shape) != (self.max_nt, self.total_n) \
+                    or vm.dtype != self.tdtype or vd.dtype != self.tdtype:
+                raise ValueError("volume source arrays do not match the batch layout")
+            qb.vol_m = vm.data_ptr()
+            qb.vol_dw = vd.data_ptr()
+            self._vol_keep = (vm, vd)
         if not image:
             qb.dw = None
             self._run(qb, 2, 0, self.max_nt, -1, 0, record=False)
         elif not self.ckpts:
-            qb.dw = self.dw.data_ptr()
+            qb.dw = img_hist.data_ptr()
             self._dw_squares(0, self.max_nt, 0)
             self._run(qb, 2, 0, self.max_nt, -1, 0, record=False)
         else:

@@ -32,22 +32,36 @@ __all__ = ["TemporalModeling", "DWaveOpHandle"]
 
 
 class DWaveOpHandle(object):
-    """dWaveOp of one imaged step of one shot, held on the device."""
+    """dWaveOp of one imaged step of one shot, held on the device: of the
+    forward field (``store`` "dw"), of the Born field ("dw1") or of the
+    adjoint field ("dw_adj")."""
 
-    __slots__ = ("batch", "shot", "k")
+    __slots__ = ("batch", "shot", "k", "store")
 
-    def __init__(self, batch, shot, k):
+    def __init__(self, batch, shot, k, store="dw"):
         self.batch = batch
         self.shot = shot
         self.k = k
+        self.store = store
 
     def __array__(self, dtype=None, copy=None):
-        a = self.batch.shot_dw(self.shot, self.k).reshape(-1, 1).double().cpu().numpy()
+        a = self.batch.shot_dw(self.shot, self.k, self.store).reshape(-1, 1).double().cpu().numpy()
         return a if dtype is None else a.astype(dtype)
 
 
-def _handles(batch, shot, nsteps, p):
-    return [DWaveOpHandle(batch, shot, k) for k in range(0, nsteps, p)]
+def _handles(batch, shot, nsteps, p, store="dw"):
+    return [DWaveOpHandle(batch, shot, k, store) for k in range(0, nsteps, p)]
+
+
+def _on_device(seq, batch=None, shot=None):
+    """The handles' (batch, shot, store) when ``seq`` is a list of device
+    handles (of ``batch``/``shot`` when given), else None."""
+    if seq is None or not len(seq) or not isinstance(seq[0], DWaveOpHandle):
+        return None
+    h = seq[0]
+    if (batch is not None and h.batch is not batch) or (shot is not None and h.shot != shot):
+        return None
+    return h.batch, h.shot, h.store
 
 
 class TemporalModeling(object):
@@ -181,43 +195,66 @@ class TemporalModeling(object):
 
     # ----------------------------------------------------------------- adjoint
     def _batch_for(self, shot, dWaveOp, imaging_period):
-        """Reuse the device history behind handles, or upload host arrays."""
+        """Reuse the device history behind handles, or upload host arrays.
+        Returns (batch, shot index in it, name of the history)."""
         s = self.solver
-        if dWaveOp is not None and len(dWaveOp) and isinstance(dWaveOp[0], DWaveOpHandle):
-            h = dWaveOp[0]
-            b = h.batch
+        dev = _on_device(dWaveOp)
+        if dev is not None:
+            b, si, store = dev
             if (b.dt == s.dt and b.max_nt == s.nsteps and b.imaging_period == imaging_period
                     and b.adjoint == self.adjoint and len(dWaveOp) == len(range(0, s.nsteps, imaging_period))):
-                return b, h.shot
+                return b, si, store
             raise ValueError("dWaveOp handles belong to another model, time axis, imaging period or adjoint mode")
         b = self.batch([shot], imaging_period)
         if dWaveOp is not None:
             b.load_dw([list(dWaveOp)])
-        return b, 0
+        return b, 0, "dw"
+
+    def _history(self, b, si, seq):
+        """A (steps, total_n) device history of ``b`` holding ``seq`` (one
+        padded array or handle per step) for shot ``si``: the store itself
+        when ``seq`` are handles of it, else an upload."""
+        dev = _on_device(seq, b, si)
+        if dev is not None:
+            hist = getattr(b, dev[2], None)
+            if hist is None:
+                raise RuntimeError(f"the {dev[2]} history behind these handles is no longer held")
+            return hist
+        return b.history_from(si, [np.asarray(a) for a in seq])
 
     def adjoint_model(self, shot, m0, operand_simdata, imaging_period=1, operand_dWaveOpAdj=None,
                       operand_model=None, return_parameters=[], dWaveOp=None, wavefield=None):
         solver = self.solver
         solver.model_parameters = m0
-        if operand_dWaveOpAdj is not None or operand_model is not None:
-            raise NotImplementedError("the full-Hessian adjoint sources (operand_dWaveOpAdj) are not supported")
-        if "dWaveOpAdj" in return_parameters:
-            raise NotImplementedError("dWaveOpAdj is not returned by the device adjoint")
         do_ic = dWaveOp is not None
         if not do_ic and "imaging_condition" in return_parameters:
             raise ValueError("To compute imaging condition, forward component must be specified.")
-        b, si = self._batch_for(shot, dWaveOp, imaging_period)
+        want_adj = "dWaveOpAdj" in return_parameters
+        # the reference adds the volume source only when both are given
+        # (_setup_adjoint_rhs, temporal_modeling.py:221-226)
+        vol = operand_dWaveOpAdj is not None and operand_model is not None
+        if (want_adj or vol) and imaging_period != 1:
+            raise ValueError("dWaveOpAdj and operand_dWaveOpAdj on the device need imaging_period 1")
+        b, si, image_store = self._batch_for(shot, dWaveOp, imaging_period)
         operands = [None] * b.n_shots
         for i in range(b.n_shots):
             operands[i] = np.zeros((b.shots[b.order_idx.index(i)].nt, b.shots[b.order_idx.index(i)].nrec))
         operands[si] = np.asarray(operand_simdata, dtype=np.float64)
+        volume = None
+        if vol:
+            # operand_model.with_padding(): zero padding (mesh.pad_array's default)
+            m1p = np.asarray(operand_model.with_padding().data)
+            volume = (b.padded_operand(si, m1p), self._history(b, si, operand_dWaveOpAdj))
         store = "adjointfield" in return_parameters
-        b.migrate(operands, store_field=store, image=do_ic)
+        b.migrate(operands, store_field=store, image=do_ic, image_store=image_store, store_adjoint_dw=want_adj,
+                  volume=volume)
         retval = dict()
         if store:
             retval["adjointfield"] = [self._unpad_field(b.shot_field(b.adjoint_store, si, k).cpu().numpy())
                                       for k in range(solver.nsteps)]
             b.adjoint_store = None
+        if want_adj:
+            retval["dWaveOpAdj"] = _handles(b, si, solver.nsteps, 1, "dw_adj")
         if do_ic:
             retval["imaging_condition"] = self._fold_images(b, -solver.dt * imaging_period, [si])
         return retval
@@ -227,15 +264,17 @@ class TemporalModeling(object):
                      wavefield=None):
         if dWaveOp is None:
             dWaveOp = self.forward_model(shot, m0, imaging_period, return_parameters=["dWaveOp"])["dWaveOp"]
-        if dWaveOpAdj is not None:
-            raise NotImplementedError("dWaveOpAdj is not returned by the device adjoint")
         rp = ["imaging_condition"]
         if adjointfield is not None:
             rp.append("adjointfield")
+        if dWaveOpAdj is not None:
+            rp.append("dWaveOpAdj")
         rv = self.adjoint_model(shot, m0, operand_simdata, imaging_period, operand_dWaveOpAdj, operand_model,
                                 return_parameters=rp, dWaveOp=dWaveOp, wavefield=wavefield)
         if adjointfield is not None:
             adjointfield[:] = rv["adjointfield"][:]
+        if dWaveOpAdj is not None:
+            dWaveOpAdj[:] = rv["dWaveOpAdj"][:]
         return rv["imaging_condition"].without_padding()
 
     def forward_traces_batch(self, shots, m0, imaging_period=1):
@@ -271,23 +310,27 @@ class TemporalModeling(object):
     def linear_forward_model(self, shot, m0, m1, return_parameters=[], dWaveOp0=None):
         solver = self.solver
         solver.model_parameters = m0
-        if "dWaveOp1" in return_parameters:
-            raise NotImplementedError("dWaveOp1 is not returned by the device Born loop")
-        if dWaveOp0 is not None and len(dWaveOp0) and isinstance(dWaveOp0[0], DWaveOpHandle):
-            b, si = self._batch_for(shot, dWaveOp0, 1)
-        elif dWaveOp0 is not None:
-            b, si = self._batch_for(shot, dWaveOp0, 1)
+        if dWaveOp0 is not None:
+            b, si, store = self._batch_for(shot, dWaveOp0, 1)
+            if store != "dw":
+                raise ValueError("dWaveOp0 must be the forward field's dWaveOp")
         else:
             b = self.batch([shot], 1)
             b.forward(keep_dw=True, segment=solver.nsteps)
             si = 0
         m1u = m1.without_padding().data.reshape(tuple(solver.mesh.shape(as_grid=True)))
-        traces = b.born(m1u)
+        keep1 = "dWaveOp1" in return_parameters
+        want_u1 = "wavefield1" in return_parameters
+        traces = b.born(m1u, store_dw=keep1, store_field=want_u1)
         retval = dict()
+        if want_u1:
+            retval["wavefield1"] = [self._unpad_field(b.shot_field(b.field_store, si, k).cpu().numpy())
+                                    for k in range(solver.nsteps)]
+            b.field_store = None
         if "simdata" in return_parameters:
             retval["simdata"] = traces[si].cpu().numpy()
         if "dWaveOp0" in return_parameters:
             retval["dWaveOp0"] = _handles(b, si, solver.nsteps, 1)
-        if "wavefield1" in return_parameters:
-            raise NotImplementedError("wavefield1 is not returned by the device Born loop")
+        if keep1:
+            retval["dWaveOp1"] = _handles(b, si, solver.nsteps, 1, "dw1")
         return retval

@@ -7,7 +7,9 @@ Mirrors pysit/objective_functions/temporal_least_squares.py:27-249:
 * compute_gradient: grad = -sum_shots migrate_shot(r), with
   aux_info 'objective_value' = 1/2 dt sum ||r||^2 and 'residual_norm'
   (:141-198)
-* apply_hessian ('approximate' / 'levenberg'): sum_shots F*F m1 (:200-249)
+* apply_hessian ('approximate' / 'levenberg'): sum_shots F*F m1; ('full'):
+  sum_shots of PySIT's two second-order migrations (:200-249, see
+  ``apply_hessian``)
 
 All shots of a call share one fused device loop per time step (the
 reference loops over shots); the per-shot results are identical to running
@@ -202,10 +204,14 @@ class TemporalLeastSquares(object):
         modes = ["approximate", "full", "levenberg"]
         if hessian_mode not in modes:
             raise ValueError(f"Invalid Hessian mode.  Valid options for applying hessian are {modes}")
-        if hessian_mode == "full":
-            raise NotImplementedError("the full Hessian needs dWaveOp1/dWaveOpAdj, not provided on the device")
         tm = self.modeling_tools
         result = m0.perturbation()
+        if hessian_mode == "full":
+            for shot in shots:
+                result += self._full_hessian_shot(shot, m0, m1)
+            if self.parallel_wrap_shot.use_parallel:
+                result = m0.perturbation(data=self._allreduce(result.asarray()))
+            return result
         for shot in shots:
             dW0 = tm.forward_model(shot, m0, 1, return_parameters=["dWaveOp"])["dWaveOp"]
             d1 = tm.linear_forward_model(shot, m0, m1, return_parameters=["simdata"], dWaveOp0=dW0)["simdata"]
@@ -215,3 +221,34 @@ class TemporalLeastSquares(object):
         if hessian_mode == "levenberg":
             result += levenberg_mu * m1
         return result
+
+    def full_hessian_terms(self, shot, m0, m1):
+        """The two migrations of PySIT's full Hessian for one shot
+        (temporal_least_squares.py:224-239), every step imaged:
+
+        * res1 = migrate_shot(r0, dWaveOp=dWaveOp1): the adjoint q of the
+          residual r0 imaged against the dWaveOp of the Born field u1 of m1,
+          keeping dWaveOpAdj1, the dWaveOp of q;
+        * res2 = migrate_shot(d1, operand_dWaveOpAdj=dWaveOpAdj1,
+          operand_model=m1, dWaveOp=dWaveOp0): the adjoint driven by the Born
+          traces d1 and the volume source m1 * dWaveOpAdj1_k.
+
+        Returns (res1, res2).  PySIT's 'full' mode is res1 + res2.  The
+        derivative of the gradient -migrate_shot(r) along m1 is res2 - res1:
+        d(dWaveOp0) = dWaveOp1 enters with the gradient's sign, and res1 is
+        built from +r0."""
+        tm = self.modeling_tools
+        # imaging period 1 throughout: dWaveOp0[k], dWaveOp1[k] and
+        # dWaveOpAdj1[k] are indexed by the time step k
+        rv = tm.forward_model(shot, m0, 1, return_parameters=["simdata", "dWaveOp"])
+        dW0 = rv["dWaveOp"]
+        r0 = shot.receivers.interpolate_data(self.solver.ts()) - rv["simdata"]
+        lin = tm.linear_forward_model(shot, m0, m1, return_parameters=["simdata", "dWaveOp1"], dWaveOp0=dW0)
+        adj1 = []
+        res1 = tm.migrate_shot(shot, m0, r0, 1, dWaveOp=lin["dWaveOp1"], dWaveOpAdj=adj1)
+        res2 = tm.migrate_shot(shot, m0, lin["simdata"], 1, operand_dWaveOpAdj=adj1, operand_model=m1, dWaveOp=dW0)
+        return res1, res2
+
+    def _full_hessian_shot(self, shot, m0, m1):
+        res1, res2 = self.full_hessian_terms(shot, m0, m1)
+        return res1 + res2

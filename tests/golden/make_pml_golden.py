@@ -30,16 +30,18 @@ Cases (closed-form inputs, no random generator in the fixture's recipe):
   40 % and 90 % of the steps, migrate_shot of the closed-form operand
   r[k, j] = sin(0.37 k + 1.3 j) exp(-k / 800), and TemporalLeastSquares'
   objective and gradient at the start model for data generated in the true
-  model; then, at the start model, linear_forward_model (Born) traces and
-  apply_hessian for the closed-form perturbation
-  m1 = 0.02 sin(7x + 1) cos(5z), and the objective history and final model
-  of three LBFGS iterations (memory 3, backtracking line search,
+  model; then, at the start model, linear_forward_model (Born) traces,
+  apply_hessian ('approximate') and the two migrations of apply_hessian
+  ('full') with the dWaveOp of the Born and of the adjoint field at the same
+  steps, for the closed-form perturbation m1 = 0.02 sin(7x + 1) cos(5z)
+  (0.02 sin(7x + 1) cos(5z) cos(3y) in 3D), and the objective history and
+  final model of three LBFGS iterations (memory 3, backtracking line search,
   optimization.py:312-501, lbfgs.py:39-124).
 * ``c3d_*`` -- a 3D layered block (x, y, z [0, 0.3], 16^3 nodes, PML(0.08,
   60) on all sides, order 4, T = 0.25, one 12 Hz Ricker, a 4 x 4 receiver
-  patch), the same quantities: traces every 2nd step, dWaveOp at 70 % of the
-  steps,
-  migrate_shot of the same closed-form operand, objective and gradient.
+  patch), the same quantities but LBFGS: traces every 2nd step, dWaveOp at
+  70 % of the steps, migrate_shot of the same closed-form operand, objective
+  and gradient, Born traces and both Hessians.
 """
 import glob
 import importlib.util
@@ -89,7 +91,7 @@ def operand(nt, nrec):
     return np.sin(0.37 * k + 1.3 * j) * np.exp(-k / 800.0)
 
 
-def run_case(ps, m, shots, solver, C, C0, every, dw_steps, second_order=False):
+def run_case(ps, m, shots, solver, C, C0, every, dw_steps, second_order=False, lbfgs=False):
     import copy
     out = {}
     true_model = solver.ModelParameters(m, {"C": C})
@@ -126,6 +128,23 @@ def run_case(ps, m, shots, solver, C, C0, every, dw_steps, second_order=False):
         dw0 = tools.forward_model(obs[0], m0, return_parameters=["dWaveOp"])["dWaveOp"]
         d1 = tools.linear_forward_model(obs[0], m0, m1, return_parameters=["simdata"], dWaveOp0=dw0)["simdata"]
         out["hessian"] = np.asarray(tools.migrate_shot(obs[0], m0, d1, 1, dWaveOp=dw0).data).reshape(-1).copy()
+        # apply_hessian('full') (temporal_least_squares.py:224-239), spelled
+        # out for the same reason: the Born field's dWaveOp1, the adjoint of
+        # the residual imaged against it keeping its own dWaveOp (dWaveOpAdj),
+        # and the adjoint of the Born traces with the volume source
+        # m1 * dWaveOpAdj_k imaged against dWaveOp0
+        dw0 = []
+        r0 = obj._residual(obs[0], m0, dWaveOp=dw0)
+        lin = tools.linear_forward_model(obs[0], m0, m1, return_parameters=["simdata", "dWaveOp1"], dWaveOp0=dw0)
+        adj1 = []
+        res1 = tools.migrate_shot(obs[0], m0, r0, 1, dWaveOp=lin["dWaveOp1"], dWaveOpAdj=adj1)
+        res2 = tools.migrate_shot(obs[0], m0, lin["simdata"], 1, operand_dWaveOpAdj=adj1, operand_model=m1,
+                                  dWaveOp=dw0)
+        out["full_res1"] = np.asarray(res1.data).reshape(-1).copy()
+        out["full_res2"] = np.asarray(res2.data).reshape(-1).copy()
+        out["dwaveop1"] = np.stack([np.asarray(lin["dWaveOp1"][k]).reshape(-1) for k in dw_steps])
+        out["dwaveopadj"] = np.stack([np.asarray(adj1[k]).reshape(-1) for k in dw_steps])
+    if lbfgs:
         invalg = ps.LBFGS(obj, memory_length=3)
         res = invalg(obs, m0, 3, line_search="backtrack", status_configuration={"objective_frequency": 1})
         out["lbfgs_objective"] = np.asarray([v for _, v in sorted(invalg.objective_history.items())])
@@ -135,6 +154,9 @@ def run_case(ps, m, shots, solver, C, C0, every, dw_steps, second_order=False):
 
 def born_direction(m):
     """A smooth closed-form model perturbation over the whole physical grid."""
+    if m.dim == 3:
+        X, Y, Z = m.mesh_coords()
+        return (0.02 * np.sin(7.0 * X + 1.0) * np.cos(5.0 * Z) * np.cos(3.0 * Y)).reshape(-1, 1)
     X, Z = m.mesh_coords()
     return (0.02 * np.sin(7.0 * X + 1.0) * np.cos(5.0 * Z)).reshape(-1, 1)
 
@@ -150,7 +172,7 @@ def c1(ps):
                                       receivers="max", receiver_depth=0.05)
     solver = ps.ConstantDensityAcousticWave(m, spatial_accuracy_order=4, trange=(0.0, 1.5),
                                             kernel_implementation="cpp")
-    return run_case(ps, m, shots, solver, C, C0, 4, [0.4, 0.9], second_order=True)
+    return run_case(ps, m, shots, solver, C, C0, 4, [0.4, 0.9], second_order=True, lbfgs=True)
 
 
 def block3d(ps):
@@ -167,7 +189,7 @@ def block3d(ps):
     shots = [ps.Shot(src, recs)]
     solver = ps.ConstantDensityAcousticWave(m, spatial_accuracy_order=4, trange=(0.0, 0.25),
                                             kernel_implementation="cpp")
-    return run_case(ps, m, shots, solver, C, C0, 2, [0.7])
+    return run_case(ps, m, shots, solver, C, C0, 2, [0.7], second_order=True)
 
 
 def main():

@@ -251,3 +251,13 @@ def golden_operand(nt, nrec):
     k = np.arange(nt)[:, None]
     j = np.arange(nrec)[None, :]
     return np.sin(0.37 * k + 1.3 * j) * np.exp(-k / 800.0)
+
+
+def golden_direction(m):
+    """The closed-form model perturbation of tests/golden/make_pml_golden.py
+    ((dof, 1), physical grid)."""
+    if m.dim == 3:
+        X, Y, Z = m.mesh_coords()
+        return (0.02 * np.sin(7.0 * X + 1.0) * np.cos(5.0 * Z) * np.cos(3.0 * Y)).reshape(-1, 1)
+    X, Z = m.mesh_coords()
+    return (0.02 * np.sin(7.0 * X + 1.0) * np.cos(5.0 * Z)).reshape(-1, 1)
