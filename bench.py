@@ -195,35 +195,39 @@ def cpu_gradient(solver, shot, seconds):
                       f"four propagations, the rate counts two)"}
 
 
-TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                            "traffic_k_step2d_layered2d.json")
+TRAFFIC_FILES = {
+    "c2": os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_k_step2d_layered2d.json"),
+    "c4": os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_k_step3d_layered3d.json"),
+}
 
 
-def measured_traffic(shots, shape, tile_x):
+def measured_traffic(shots, shape, tile_x, workload="c2"):
     """HBM bytes per launch of the forward and adjoint kernels from the
     committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE per
     dispatch, each scaled by the factor the same counter needs on a copy of
     known size), when it was measured on this workload and tile height;
     else nulls."""
+    path = TRAFFIC_FILES[workload]
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return {}
     if (t.get("shots") != shots or list(t.get("grid_padded", [])) != list(shape)
             or t.get("tile_x") != tile_x):
         return {}
-    out = {"source": os.path.relpath(TRAFFIC_FILE, os.path.dirname(TRAFFIC_FILE) + "/..")}
+    out = {"source": os.path.relpath(path, os.path.dirname(path) + "/..")}
     for k in ("forward", "adjoint"):
         d = t["kernels"][k]
         out[k] = d["fetch_kib"] * 1024 * t["fetch_factor"] + d["write_kib"] * 1024 * t["write_factor"]
     return out
 
 
-def write_traffic(fetch_csv, write_csv, shots=16, shape=(1041, 341)):
-    """profiles/traffic_k_step2d_layered2d.json from the counter files of
-    the two `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs of
-    `bench.py --pmc-run`: per-dispatch averages of the forward (mode 1, with
+def write_traffic(fetch_csv, write_csv, workload="c2"):
+    """profiles/traffic_k_step2d_layered2d.json (c2) or
+    traffic_k_step3d_layered3d.json (c4) from the counter files of the two
+    `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs of `bench.py
+    --pmc-run`: per-dispatch averages of the forward (mode 1, with
     receivers: the largest grid) and adjoint (mode 4) kernels of the tile
     height most dispatched, and the factors that make each counter read the
     calibration copy's 262144 KiB (torch.add over 2^26 floats)."""
@@ -241,7 +245,12 @@ def write_traffic(fetch_csv, write_csv, shots=16, shape=(1041, 341)):
             d["v"] += float(r["Counter_Value"])
         return list(disp.values())
 
-    pat = re.compile(r"pa::k_step2d<float, (\d+), 4, (\d), false>")
+    if workload == "c4":
+        shots, shape, kname = 8, (288, 288, 288), "k_step3d"
+        pat = re.compile(r"pa::k_step3d<float, 4, (\d), true>")
+    else:
+        shots, shape, kname = 16, (1041, 341), "k_step2d"
+        pat = re.compile(r"pa::k_step2d<float, (\d+), 4, (\d), false>")
     out = {}
     for key, path, counter in (("fetch", fetch_csv, "FETCH_SIZE"), ("write", write_csv, "WRITE_SIZE")):
         ds = load(path, counter)
@@ -249,7 +258,9 @@ def write_traffic(fetch_csv, write_csv, shots=16, shape=(1041, 341)):
         step = collections.defaultdict(list)
         for d in ds:
             m = pat.search(d["name"])
-            if m:
+            if m and workload == "c4":
+                step[(16, int(m.group(1)))].append(d)
+            elif m:
                 step[(int(m.group(1)), int(m.group(2)))].append(d)
         tx = max({k[0] for k in step}, key=lambda t: sum(len(v) for k, v in step.items() if k[0] == t))
         fwd = step[(tx, 1)]
@@ -260,28 +271,33 @@ def write_traffic(fetch_csv, write_csv, shots=16, shape=(1041, 341)):
                     "adj": (sum(adj) / len(adj), len(adj))}
     f, w = out["fetch"], out["write"]
     tx = f["tx"]
+    wflag = " --workload c4" if workload == "c4" else ""
     doc = {
-        "workload": f"2D layered, {shape[0]}x{shape[1]} padded, {shots} shots, order 8, FP32 "
-                    "(bench.py --pmc-run, 40 slots)",
+        "workload": (f"3D layered, {'x'.join(map(str, shape))} padded, {shots} shots, order 8, FP32 "
+                     "(bench.py --workload c4 --pmc-run, 12 slots)" if workload == "c4" else
+                     f"2D layered, {shape[0]}x{shape[1]} padded, {shots} shots, order 8, FP32 "
+                     "(bench.py --pmc-run, 40 slots)"),
         "shots": shots, "grid_padded": list(shape), "tile_x": tx,
-        "commands": [
-            "PYSIT_AMD_TUNE_CACHE=gpurun_out/tune.json python3 bench.py  (the launch configuration chosen)",
+        "commands": ([] if workload == "c4" else [
+            "PYSIT_AMD_TUNE_CACHE=gpurun_out/tune.json python3 bench.py  (the launch configuration chosen)"]) + [
             "PYSIT_AMD_TUNE_CACHE=gpurun_out/tune.json rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmcf -o run "
-            "--output-format csv -- python3 bench.py --pmc-run --steps 1 --warmup 0",
+            f"--output-format csv -- python3 bench.py{wflag} --pmc-run --steps 1 --warmup 0",
             "PYSIT_AMD_TUNE_CACHE=gpurun_out/tune.json rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmcw -o run "
-            "--output-format csv -- python3 bench.py --pmc-run --steps 1 --warmup 0",
-            "python3 bench.py --write-traffic gpurun_out/pmcf/run_counter_collection.csv "
+            f"--output-format csv -- python3 bench.py{wflag} --pmc-run --steps 1 --warmup 0",
+            f"python3 bench.py{wflag} --write-traffic gpurun_out/pmcf/run_counter_collection.csv "
             "gpurun_out/pmcw/run_counter_collection.csv"],
         "calibration": {"kernel": "torch.add(x, 1.0, out=y) over 2^26 float32 (reads and writes 262144 KiB)",
                         "fetch_kib": f["cal"], "write_kib": w["cal"]},
         "fetch_factor": round(262144.0 / f["cal"], 4), "write_factor": round(262144.0 / w["cal"], 4),
         "kernels": {
-            "forward": {"name": f"k_step2d<float,{tx},4,1>", "dispatches": f["fwd"][1],
+            "forward": {"name": f"k_step3d<float,4,1>" if workload == "c4" else f"k_step2d<float,{tx},4,1>",
+                        "dispatches": f["fwd"][1],
                         "fetch_kib": round(f["fwd"][0], 1), "write_kib": round(w["fwd"][0], 1)},
-            "adjoint": {"name": f"k_step2d<float,{tx},4,4>", "dispatches": f["adj"][1],
+            "adjoint": {"name": f"k_step3d<float,4,4>" if workload == "c4" else f"k_step2d<float,{tx},4,4>",
+                        "dispatches": f["adj"][1],
                         "fetch_kib": round(f["adj"][0], 1), "write_kib": round(w["adj"][0], 1)}},
     }
-    with open(TRAFFIC_FILE, "w") as fh:
+    with open(TRAFFIC_FILES[workload], "w") as fh:
         json.dump(doc, fh, indent=1)
         fh.write("\n")
     print(json.dumps(doc))
@@ -369,6 +385,16 @@ def run_3d(args):
     from pysit_amd.device import DeviceBatch
     from pysit_amd.modeling import TemporalModeling
     from pysit_amd.workloads import layered_3d
+    if args.pmc_run:
+        # calibration: y = x + 1 over 2^26 floats reads and writes exactly 256 MiB
+        x = torch.ones(2 ** 26, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.add(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        del x, y
+        if args.nt == 3000:
+            args.nt = 12
     t0 = time.perf_counter()
     nt = args.nt if args.nt != 3000 else 2000
     mesh, shots, solver, model = layered_3d(n=256, n_shots=8, nt=nt)
@@ -408,11 +434,18 @@ def run_3d(args):
     b = DeviceBatch(geos, model_u, mesh.deltas, solver.dt, 8, dtype=np.float32, memory_budget=budget)
     b32_window = bool(b.ic_window)
     res = [t.clone() for t in b.forward(keep_dw=False)]
+    if args.pmc_run:
+        b.forward(keep_dw=True)
+        b.migrate(res)
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_run": "done", "nt": nt, "shots": len(shots), "tile_x": b.tile_x}), flush=True)
+        return
     t32 = gradient(b, res, True)
     timer = b.timer
     adj_us, n_adj = avg_us(timer, 2)
     fwd_us, n_fwd = avg_us(timer, 1)
     b_fwd, b_adj = algorithmic_bytes([(npts, npml)] * len(shots), 4, dim=3)
+    traffic = measured_traffic(len(shots), shape, b.tile_x, workload="c4")
     K = b.K
     img32 = b.image(0).cpu().numpy().reshape(-1).copy()
     tr32 = res[0].cpu().numpy().copy()
@@ -450,13 +483,14 @@ def run_3d(args):
                    "shots": len(shots), "nt": nt, "grid_padded": list(shape), "dwaveop_segment": K,
                    "checkpoints": -(-nt // K)},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic.get("adjoint"),
                      "kernel": ("k_step3d<float,4,4> (adjoint + imaging, FP32 window sums into FP64)"
                                 if b32_window else "k_step3d<float,4,2> (adjoint + FP64 imaging)"),
                      "bytes_per_launch": b_adj, "avg_launch_us": adj_us, "launches": n_adj},
         "forward_kernel": {"kernel": "k_step3d<float,4,1> (recompute + dWaveOp store)", "avg_launch_us": fwd_us,
                            "achieved_gbs": b_fwd / (fwd_us * 1e-6) / 1e9, "bytes_per_launch": b_fwd,
-                           "launches": n_fwd},
+                           "launches": n_fwd, "traffic": traffic.get("forward")},
+        "traffic_source": traffic.get("source"),
         "gradient_roofline_frac": value * 1e9 * (b_fwd + b_adj) / (2 * npts * len(shots)) / 1e9 / HBM_PEAK_GBS,
         "accuracy_vs_fp64": acc, "host_setup_s": t_setup}), flush=True)
 
@@ -530,7 +564,7 @@ def run_fwi(args):
 def main():
     args = parse()
     if args.write_traffic:
-        write_traffic(*args.write_traffic)
+        write_traffic(*args.write_traffic, workload=args.workload)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (shots shard across GPUs "

@@ -148,9 +148,14 @@ typedef struct {
     const pysit_amd_shot *shots;
     /* tiles: int4 {shot, tile index t within the shot, flags, x tile index
      * t / (tiles per x row)}; flags bit0 = the tile is within `order/2` nodes
-     * of a PML node (full path); 2D bit1 = no row of the tile or within
-     * `order/2` rows of it is an x PML row or an x edge row.  Shots sorted so that, for every slot, the
-     * active tiles form a prefix.                                            */
+     * of a PML node (full path); bit1 = no row (2D) or x plane (3D) of the
+     * tile or within `order/2` of it is an x PML row/plane or an x edge;
+     * 3D bit2 = no node of the tile or within `order/2` nodes of it lies in
+     * two or more PML bands: the full path then neither reads nor advances
+     * psi, which enters only multiplied by products of two or three sigmas
+     * (a bit that is wrongly set gives wrong results; clear is always
+     * correct).  Shots sorted so that, for every slot, the active tiles form
+     * a prefix.                                                              */
     int n_tiles;
     const int *tiles;
     /* injection, two-level CSR: tile t owns destination nodes

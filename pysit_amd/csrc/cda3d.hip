@@ -64,6 +64,14 @@ int tile_rows_3d(int axis) { return axis == 0 ? TX3 : TY3; }
 #ifndef PA3D_PREFETCH
 #define PA3D_PREFETCH 1
 #endif
+// x planes in flight in PML tiles that touch at most one PML band (they
+// keep no psi, below): 1 (2 measured 2-6 % slower on the 3D steps)
+#ifndef PA3D_PREFETCH_NOPSI
+#define PA3D_PREFETCH_NOPSI 1
+#endif
+#ifndef PA3D_PSI_SKIP
+#define PA3D_PSI_SKIP 1
+#endif
 #ifndef PA3D_PREFETCH_INT
 #define PA3D_PREFETCH_INT 2
 #endif
@@ -277,11 +285,21 @@ __device__ __forceinline__ void march3_int(const Ctx3D<T, R> &c, const Step3DPar
 // XC: no plane of the tile or within R planes of it is an x PML plane or an
 // x edge (the host flags such tiles): sigma_x is zero on every plane, the
 // terms it multiplies drop out and the damping factor is one per lane.
-template <typename T, int R, int MODE, bool XC, bool INC>
+// PSI: some node of the tile or within R nodes of it lies in two or more PML
+// bands.  psi only enters multiplied by a product of two or three sigmas:
+// the node's own psi by sigma_x sigma_y sigma_z, and the psi derivative
+// along one axis by the product of the other two axes' sigmas at the node
+// taking it, whose stencil nodes share those two coordinates, hence those
+// two bands.  So psi is read only at nodes in two or more bands, and a tile
+// without such a node (PSI false) neither reads nor advances it: its psi
+// stays zero, every term it would enter is an exact zero, and the results
+// equal the full form's (at most the sign of a zero differs).
+template <typename T, int R, int MODE, bool XC, bool INC, bool PSI>
 __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DParams<T> &P, Lds3D<T, R> &L,
                                             int x0, int hl_pz)
 {
-    constexpr int TX = TX3, D = PA3D_PREFETCH, NQ = TX + 2 * R, LZ = Lds3D<T, R>::LZ;
+    constexpr int TX = TX3, D = PSI ? PA3D_PREFETCH : PA3D_PREFETCH_NOPSI, NQ = TX + 2 * R;
+    constexpr int LZ = Lds3D<T, R>::LZ;
     constexpr int SZ = sizeof(T);
     const int pl = c.plane, nx = c.nx;
     const int co = c.co, cho = c.cho, hro = c.hro, ov = c.ov, mv = c.mv;
@@ -298,7 +316,7 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
         const bool xb = c.gen || xx < c.nxl || xx >= nx - c.nxr;                                    \
         q[s_] = buf_ld<T>(c.u, o, 0);                                                               \
         qx[s_] = buf_ld<T>(c.px, (xb || c.bp) ? o : kOOB, 0);                                       \
-        qs[s_] = buf_ld<T>(c.ps, o, 0);                                                             \
+        qs[s_] = PSI ? buf_ld<T>(c.ps, o, 0) : T(0);                                                \
     } while (0)
 #define PA_ROW(r_)                                                                                  \
     do {                                                                                            \
@@ -310,8 +328,8 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
         const bool xb = c.gen || x < c.nxl || x >= nx - c.nxr;                                      \
         uz[r] = buf_ld<T>(c.u, oz, 0);                                                              \
         yh[r] = buf_ld<T>(c.u, oh, 0);                                                              \
-        sz[r] = buf_ld<T>(c.ps, oz, 0);                                                             \
-        sh[r] = buf_ld<T>(c.ps, oh, 0);                                                             \
+        sz[r] = PSI ? buf_ld<T>(c.ps, oz, 0) : T(0);                                                \
+        sh[r] = PSI ? buf_ld<T>(c.ps, oh, 0) : T(0);                                                \
         pyc[r] = buf_ld<T>(c.py, (xb || c.bp) ? oc : kOOB, 0);                                      \
         pyh[r] = buf_ld<T>(c.py, (xb || c.bph) ? oh : kOOB, 0);                                     \
         pzc[r] = buf_ld<T>(c.pz, (xb || c.bp) ? oc : kOOB, 0);                                      \
@@ -347,15 +365,15 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
         T *lu = L.u[b], *lps = L.ps[b], *lpy = L.py[b], *lpz = L.pz[b];
         const T uc = q[R + j], psc = qs[R + j];
         lu[own] = uc;
-        lps[own] = psc;
+        if constexpr (PSI) lps[own] = psc;
         lpy[owny] = pyc[j];
         lpz[ownz] = pzc[j];
         st_halo<T, R>(lu, L.junk[0], c.hl, c.lane, uz[j]);
-        st_halo<T, R>(lps, L.junk[1], c.hl, c.lane, sz[j]);
+        if constexpr (PSI) st_halo<T, R>(lps, L.junk[1], c.hl, c.lane, sz[j]);
         st_halo<T, R>(lpz, L.junk[2], hl_pz, c.lane, pzz[j]);
         if (c.hrow >= 0) {
             lu[c.hrow * LZ + R + c.lane] = yh[j];
-            lps[c.hrow * LZ + R + c.lane] = sh[j];
+            if constexpr (PSI) lps[c.hrow * LZ + R + c.lane] = sh[j];
             lpy[c.hrow * 64 + c.lane] = pyh[j];
         }
         lds_barrier();
@@ -376,17 +394,19 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
             const T ax = P.cx.d1[i - 1], ay = P.cy.d1[i - 1], az = P.cz.d1[i - 1];
             dux = fma(ax, q[R + j + i] - q[R + j - i], dux);
             dpx = fma(ax, qx[R + j + i] - qx[R + j - i], dpx);
-            dsx = fma(ax, qs[R + j + i] - qs[R + j - i], dsx);
             duy = fma(ay, lu[own + i * LZ] - lu[own - i * LZ], duy);
             dpy = fma(ay, lpy[owny + i * 64] - lpy[owny - i * 64], dpy);
-            dsy = fma(ay, lps[own + i * LZ] - lps[own - i * LZ], dsy);
             duz = fma(az, lu[own + i] - lu[own - i], duz);
             dpz = fma(az, lpz[ownz + i] - lpz[ownz - i], dpz);
-            dsz = fma(az, lps[own + i] - lps[own - i], dsz);
+            if constexpr (PSI) {
+                dsx = fma(ax, qs[R + j + i] - qs[R + j - i], dsx);
+                dsy = fma(ay, lps[own + i * LZ] - lps[own - i * LZ], dsy);
+                dsz = fma(az, lps[own + i] - lps[own - i], dsz);
+            }
         }
         const T src = (((f + lap) + dpx) + dpy) + dpz;
         const T dt = P.tc.dt;
-        T psn = psc + dt * uc;
+        T psn = PSI ? psc + dt * uc : T(0);
         const T pxc = qx[R + j], pycj = pyc[j], pzcj = pzc[j];
         T s1k, s2k, s1m, s2m, srcp, pxd, pyd, pzd, dws;
         bool pml;
@@ -401,7 +421,8 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
             s2m = sym * szm;
             pml = syk != T(0) || szk != T(0);
             srcp = src;
-            pxd = pxc + dt * (syk + szk) * dux + dt * (syk * szk * dsx);
+            pxd = pxc + dt * (syk + szk) * dux;
+            if constexpr (PSI) pxd = pxd + dt * (syk * szk * dsx);
             pyd = (pycj - dt * syk * pycj) + dt * (szk - syk) * duy;
             pzd = (pzcj - dt * szk * pzcj) + dt * (syk - szk) * duz;
             dws = T(0);
@@ -411,14 +432,23 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
             s2k = (sxk * syk + syk * szk) + szk * sxk;
             const T s3k = (sxk * syk) * szk;
             pml = sxk != T(0) || syk != T(0) || szk != T(0);
-            srcp = src - s3k * psc;
-            pxd = (pxc - dt * sxk * pxc) + dt * ((syk + szk) - sxk) * dux + dt * (syk * szk * dsx);
-            pyd = (pycj - dt * syk * pycj) + dt * ((szk + sxk) - syk) * duy + dt * (szk * sxk * dsy);
-            pzd = (pzcj - dt * szk * pzcj) + dt * ((sxk + syk) - szk) * duz + dt * (sxk * syk * dsz);
+            pxd = (pxc - dt * sxk * pxc) + dt * ((syk + szk) - sxk) * dux;
+            pyd = (pycj - dt * syk * pycj) + dt * ((szk + sxk) - syk) * duy;
+            pzd = (pzcj - dt * szk * pzcj) + dt * ((sxk + syk) - szk) * duz;
             s1m = (sxm + sym) + szm;
             s2m = (sxm * sym + sxm * szm) + sym * szm;
-            const T s3m = (sxm * sym) * szm;
-            dws = s3m * psc;
+            if constexpr (PSI) {
+                srcp = src - s3k * psc;
+                pxd = pxd + dt * (syk * szk * dsx);
+                pyd = pyd + dt * (szk * sxk * dsy);
+                pzd = pzd + dt * (sxk * syk * dsz);
+                const T s3m = (sxm * sym) * szm;
+                dws = s3m * psc;
+            } else {
+                (void)s3k;
+                srcp = src;
+                dws = T(0);
+            }
         }
         const T fac1 = P.tc.two_dt2 / (T(2) + dt * s1k);
         const T S = c2 * srcp;
@@ -441,7 +471,7 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
             up = pml ? upd : upi;
             dw = S + (s1m - s1k) * (up - u_m) * P.tc.inv_2dt + (s2m - s2k) * uc;
         }
-        if constexpr (!XC) dw = dw + dws;
+        if constexpr (!XC && PSI) dw = dw + dws;
         if (edge) {
             up = T(0);
             dn = T(0);
@@ -462,7 +492,7 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
         buf_st(c.pxo, op, 0, pxn);
         buf_st(c.pyo, op, 0, pyn);
         buf_st(c.pzo, op, 0, pzn);
-        buf_st(c.pso, oo, 0, psn);
+        if constexpr (PSI) buf_st(c.pso, oo, 0, psn);
         if (MODE == 1) buf_st(c.dw, inx ? at(ovd, xp) : kOOB, c.shb, dw);
         if (out3(MODE)) buf_st(c.dwo, inx ? at(ovo, xp) : kOOB, c.shb, dw);
         if (c.has_fst) buf_st(c.fst, inx ? at(ovf, xp) : kOOB, c.shb, uc);
@@ -473,7 +503,8 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
 
 template <typename T, int R, int MODE, bool INC>
 __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DParams<T> &P,
-                                       const pysit_amd_shot &S, int tile, int full, int xclear, int tloc, int slot,
+                                       const pysit_amd_shot &S, int tile, int full, int xclear, int psi, int tloc,
+                                       int slot,
                                        int seg_base, Lds3D<T, R> &L)
 {
     constexpr int SZ = sizeof(T), TX = TX3, LZ = Lds3D<T, R>::LZ;
@@ -628,8 +659,10 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
     }
     if (!has_inj && threadIdx.x == 0) L.junk[3][0] = T(0);
     lds_barrier();
-    if (full && xclear) march3_full<T, R, MODE, true, INC>(c, P, L, x0, hl_pz);
-    else if (full) march3_full<T, R, MODE, false, INC>(c, P, L, x0, hl_pz);
+    if (full && xclear && psi) march3_full<T, R, MODE, true, INC, true>(c, P, L, x0, hl_pz);
+    else if (full && xclear) march3_full<T, R, MODE, true, INC, false>(c, P, L, x0, hl_pz);
+    else if (full && psi) march3_full<T, R, MODE, false, INC, true>(c, P, L, x0, hl_pz);
+    else if (full) march3_full<T, R, MODE, false, INC, false>(c, P, L, x0, hl_pz);
     else march3_int<T, R, MODE, INC>(c, P, L, x0);
 }
 
@@ -683,10 +716,12 @@ void k_step3d(const pysit_amd_batch B, const Step3DParams<T> P,
     const int blk = xcd_block(b, tile_blocks);
     if (blk >= n_tile_blocks) return;
     if (!INC) {
-        tile3d<T, R, MODE, INC>(B, P, P.shot0, blk, 1, 0, blk, slot, seg_base, L);
+        tile3d<T, R, MODE, INC>(B, P, P.shot0, blk, 1, 0, 1, blk, slot, seg_base, L);
     } else {
         const int4 td = reinterpret_cast<const int4 *>(B.tiles)[blk];
-        tile3d<T, R, MODE, INC>(B, P, B.shots[td.x], blk, td.z & 1, (td.z >> 1) & 1, td.y, slot, seg_base, L);
+        // flags bit2: the tile and its reach touch at most one PML band (no psi)
+        const int psi = PA3D_PSI_SKIP ? ((td.z >> 2) & 1) ^ 1 : 1;
+        tile3d<T, R, MODE, INC>(B, P, B.shots[td.x], blk, td.z & 1, (td.z >> 1) & 1, psi, td.y, slot, seg_base, L);
     }
 }
 

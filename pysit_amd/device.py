@@ -306,17 +306,30 @@ class DeviceBatch:
         nty = -(-ny // self.tile_y)
         ntx = -(-nx // self.tile_x)
         (xl, xr), (yl, yr), (zl, zr) = s.kpml
+
+        def band(lo, hi, n, left, right):
+            # the node range [lo - R, hi + R] meets a PML band of the axis
+            return (left.size > 0 and lo - R < left.size) or (right.size > 0 and hi + R > n - 1 - right.size)
+
         flags = np.zeros(ntx * nty * ntz, dtype=np.int32)
         for tx in range(ntx):
             x0, x1 = tx * self.tile_x, min(nx, (tx + 1) * self.tile_x) - 1
             xp = (x0 - R < max(xl.size, 1)) or (x1 + R > nx - 1 - max(xr.size, 1))
+            xb = band(x0, x1, nx, xl, xr)
             for ty in range(nty):
                 y0, y1 = ty * self.tile_y, min(ny, (ty + 1) * self.tile_y) - 1
                 yp = (y0 - R < max(yl.size, 1)) or (y1 + R > ny - 1 - max(yr.size, 1))
+                yb = band(y0, y1, ny, yl, yr)
                 for tz, (z0, zhi) in enumerate(zt):
                     zp = (z0 - R < max(zl.size, 1)) or (zhi - 1 + R > nz - 1 - max(zr.size, 1))
-                    # bit0: the full path; bit1: its planes need no x PML form
-                    flags[(tx * nty + ty) * ntz + tz] = (1 if (xp or yp or zp) else 0) | (0 if xp else 2)
+                    zb = band(z0, zhi - 1, nz, zl, zr)
+                    # bit0: the full path; bit1: its planes need no x PML
+                    # form; bit2: the tile's reach meets at most one PML band,
+                    # so no node it reads or writes lies in two (the box it
+                    # reaches is a product of ranges, the bands are slabs):
+                    # psi is neither read nor advanced there
+                    flags[(tx * nty + ty) * ntz + tz] = ((1 if (xp or yp or zp) else 0) | (0 if xp else 2)
+                                                         | (4 if int(xb) + int(yb) + int(zb) <= 1 else 0))
         return (ntx, nty, ntz), flags
 
     def _tile_of_pos(self, s, s_tiles, pos):
@@ -702,27 +715,41 @@ class DeviceBatch:
         if not keep_dw:
             return self.max_nt
         budget = self.memory_budget
-        if budget is None:
-            free, _ = torch.cuda.mem_get_info(self.dev)
-            unused = torch.cuda.memory_reserved(self.dev) - torch.cuda.memory_allocated(self.dev)
-            held = self.dw.numel() * self.dw.element_size() if getattr(self, "dw", None) is not None else 0
-            held += sum(v.numel() * v.element_size() for c in getattr(self, "ckpts", []) for v in c.values())
-            budget = int(0.9 * (free + unused + held))
         item = self.dtype.itemsize
         n_inj = int(self.adj_inj[2].numel()) - 1 if self.adj_inj is not None else 0
         per_slot = (self.total_n + n_inj) * item
         ckpt = self.ckpt_bytes()
-        # what migrate allocates beside the store: the adjoint state (unless
-        # held already), the working state of the replays and the FP64 and
-        # FP32 imaging accumulators
         state = self.state_bytes()
-        budget -= (0 if getattr(self, "qstate", None) is not None else state) + 12 * self.total_n
+        if budget is None:
+            # everything this batch holds from an earlier gradient counts as
+            # available (a new one reuses or releases it), so repeated calls
+            # plan the same schedule and keep their buffers
+            free, _ = torch.cuda.mem_get_info(self.dev)
+            unused = torch.cuda.memory_reserved(self.dev) - torch.cuda.memory_allocated(self.dev)
+            mine = [getattr(self, "dw", None), getattr(self, "_inj_pre", None), getattr(self, "ic", None),
+                    getattr(self, "_ic32", None)]
+            mine += [v for c in getattr(self, "ckpts", []) for v in c.values()]
+            held = sum(v.numel() * v.element_size() for v in mine if v is not None)
+            held += state if getattr(self, "qstate", None) is not None else 0
+            budget = int(0.9 * (free + unused + held))
+        # what migrate allocates beside the store: the adjoint state, the
+        # working state of the replays and the FP64 and FP32 imaging
+        # accumulators
+        budget -= state + 12 * self.total_n
         self.levels = [self.max_nt]
         if self.max_nt * per_slot <= budget:
             return self.max_nt
         levels = checkpoint_levels(self.max_nt, per_slot, ckpt, budget, state=state)
         if levels is None:
             raise MemoryError("memory budget too small for the checkpoint schedule")
+        held = getattr(self, "dw_slots", 0) if getattr(self, "dw", None) is not None else 0
+        if (self.memory_budget is None and len(levels) == 1 and held and held == getattr(self, "K", 0)
+                and abs(levels[0] - held) <= max(2, levels[0] // 20)):
+            # the earlier gradient ran this schedule with the store it still
+            # holds: keep it rather than reallocate for a few slots of
+            # difference in the free-memory estimate (a reallocation of a
+            # store of hundreds of GB takes seconds)
+            levels = [held]
         self.levels = levels
         return levels[0]
 

@@ -148,6 +148,18 @@ def test_supershot_traces_are_the_coded_sum(gpu):
     assert f <= 1e-20 * 0.5 * solver.dt * np.linalg.norm(want) ** 2
     g = obj.compute_gradient([sup], start_model)
     assert np.all(np.isfinite(np.asarray(g.data))) and np.linalg.norm(np.asarray(g.data)) > 0
+    # the supershot's gradient against the oracle: its forward with the
+    # encoded sources (one channel per member source, each scaled by its
+    # code), the encoded data's residual, and the oracle's migration of it
+    from tests.problems import oracle_from_solver, rel_l2
+    solver.model_parameters = start_model
+    op = oracle_from_solver(solver, sup)
+    r = sup.receivers.interpolate_data(solver.ts()) - op.forward()["simdata"]
+    assert rel_l2(g.data.reshape(-1), -op.migrate(r)) < 1e-10
+    # and the oracle's encoded forward is the code-weighted sum of its
+    # member shots' forwards
+    members = sum(c * oracle_from_solver(solver, sh).forward()["simdata"] for c, sh in zip(sup.codes, shots))
+    assert rel_l2(op.forward()["simdata"], members) < 1e-12
 
 
 @pytest.mark.parametrize("method", ["savemat", "pickle", "npy"])
