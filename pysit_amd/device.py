@@ -404,17 +404,8 @@ class DeviceBatch:
             self._tile_lookup.append(look)
         self.tiles = torch.from_numpy(tiles.reshape(-1).copy()).to(dev)
 
-        # active prefix per slot
-        shot_tiles = np.array([len(ti[1]) for ti in tiles_info])
-        shot_recs = np.array([s.nrec for s in self.shots])
-        ks = np.array(self.kshift)
-        slots = np.arange(self.max_nt)
-        active = ks[None, :] <= slots[:, None]
-        if same:
-            self.active_tiles = np.full(self.max_nt, self.n_tiles, dtype=np.int32)
-        else:
-            self.active_tiles = (active * shot_tiles[None, :]).sum(1).astype(np.int32)
-        self.active_recs = (active * shot_recs[None, :]).sum(1).astype(np.int32)
+        self._same = same
+        self._active_prefix()
 
         # PML arrays and model windows (shared tensors where equal)
         self._keep = []
@@ -457,6 +448,70 @@ class DeviceBatch:
         self.shots_fwd = self._shot_table(self.amp_dev, [s.amp.shape[1] for s in self.shots])
         self.residual_dev = None
         self.shots_adj = None
+
+    def _active_prefix(self):
+        """Active tiles and receivers per slot (shots start at their kshift)."""
+        shot_tiles = np.array([len(ti[1]) for ti in self.tiles_info])
+        shot_recs = np.array([s.nrec for s in self.shots])
+        ks = np.array(self.kshift)
+        slots = np.arange(self.max_nt)
+        active = ks[None, :] <= slots[:, None]
+        if self._same:
+            self.active_tiles = np.full(self.max_nt, self.n_tiles, dtype=np.int32)
+        else:
+            self.active_tiles = (active * shot_tiles[None, :]).sum(1).astype(np.int32)
+        self.active_recs = (active * shot_recs[None, :]).sum(1).astype(np.int32)
+
+    def retime(self, shots, dt):
+        """Take a new time axis in place: ``shots`` are the batch's shots (in
+        the caller's order) with new step counts and amplitudes at step
+        ``dt``, and everything else equal (grids, PML, model windows,
+        injection and receiver entries).  The tables, the launch
+        configuration and the state are kept; the histories of the old time
+        axis (dWaveOp stores, checkpoints, traces) are dropped.  A trial
+        model of an inversion changes dt with its largest speed: retiming
+        costs what the new amplitudes cost instead of a rebuilt batch.
+        Raises ValueError when the shots differ otherwise or their order by
+        length changes (build a new batch then)."""
+        if len(shots) != self.n_shots:
+            raise ValueError("retime needs the batch's shots")
+        max_nt = max(s.nt for s in shots)
+        kshift = [max_nt - s.nt for s in shots]
+        order = sorted(range(len(shots)), key=lambda i: kshift[i])
+        if order != self.order_idx or (self._same and len(set(kshift)) != 1):
+            raise ValueError("the shots' order by length changes")
+        new = [shots[i] for i in order]
+
+        def same_entries(a, b):
+            return a is b or (a is not None and b is not None and all(np.array_equal(x, y) for x, y in zip(a, b)))
+
+        for a, b in zip(new, self.shots):
+            if (a.shape != b.shape or a.window != b.window or a.nrec != b.nrec
+                    or not all(np.array_equal(x, y) for p, q in zip(a.kpml, b.kpml) for x, y in zip(p, q))
+                    or not same_entries(a.src, b.src) or not same_entries(a.rec, b.rec)
+                    or not same_entries(a.adj, b.adj)):
+                raise ValueError("retime changes only the time axis of the same shots")
+        self.generation += 1
+        self.dt = float(dt)
+        self.max_nt = max_nt
+        self.kshift = [kshift[i] for i in order]
+        self.shots = new
+        self._active_prefix()
+        self.amp_dev = [torch.from_numpy(s.amp).to(self.dev) for s in self.shots]
+        self.traces = [torch.zeros((s.nt, s.nrec), dtype=torch.float64, device=self.dev) for s in self.shots]
+        self.shots_fwd = self._shot_table(self.amp_dev, [s.amp.shape[1] for s in self.shots])
+        self.residual_dev = None
+        self.shots_adj = None
+        # histories of the old time axis (the store is kept for reuse: it is
+        # indexed by slot, so a store of at least the slots a run needs serves)
+        self.ckpts = []
+        self._dw_seg = None
+        self.have_dw = False
+        self.dw1 = None
+        self.dw_adj = None
+        self.field_store = None
+        self.adjoint_store = None
+        self.K = self.max_nt
 
     def _model_layout(self):
         """The kernels' copy of the model.  When every shot takes the same z
@@ -756,8 +811,13 @@ class DeviceBatch:
     def _ensure_dw(self, K):
         if self.dw is None or self.dw_slots < K:
             self.dw = None
-            torch.cuda.empty_cache()
-            self.dw = torch.empty((K, self.total_n), dtype=self.tdtype, device=self.dev)
+            self.dw_slots = 0
+            try:
+                self.dw = torch.empty((K, self.total_n), dtype=self.tdtype, device=self.dev)
+            except torch.cuda.OutOfMemoryError:
+                # the released store may still sit in torch's cache
+                torch.cuda.empty_cache()
+                self.dw = torch.empty((K, self.total_n), dtype=self.tdtype, device=self.dev)
             self.dw_slots = K
 
     # ----------------------------------------------------------------- forward

@@ -128,6 +128,18 @@ class TemporalModeling(object):
             b = cached[1]
             b.set_model(s.unpadded_model())
             return b
+        if reuse and cached is not None and cached[0][:1] + cached[0][3:] == key[:1] + key[3:]:
+            # the same shots on another time axis (a trial model with another
+            # largest speed): new amplitudes and step counts, same tables
+            b = cached[1]
+            try:
+                b.retime([self.geometry(sh) for sh in shots], s.dt)
+            except ValueError:
+                b = None
+            if b is not None:
+                b.set_model(s.unpadded_model())
+                self._batch_cache = (key, b, list(shots))
+                return b
         if reuse:
             self._batch_cache = None
         b = self.make_batch([self.geometry(sh) for sh in shots], s.unpadded_model(), imaging_period)

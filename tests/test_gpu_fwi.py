@@ -50,6 +50,39 @@ def test_objective_reuses_its_batch_across_models(gpu):
     assert np.linalg.norm(d) <= 1e-13 * np.linalg.norm(np.asarray(g1_ref.data))
 
 
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_objective_retimes_its_batch_for_a_new_time_axis(gpu, precision):
+    """A trial model with a larger maximum speed has a shorter dt and more
+    steps: evaluate / compute_gradient retime the cached batch (same tables,
+    new amplitudes and step counts) and give bit for bit what a fresh
+    objective gives; the time axis can change back again."""
+    import pysit_amd as pa
+    mesh, solver, true_model, start_model = _small_fwi(precision)
+    shots = _shots(mesh, 8.0)
+    pa.generate_seismic_data(shots, solver, true_model)
+    obj = pa.TemporalLeastSquares(solver)
+    f0 = obj.evaluate(shots, start_model)
+    b0 = obj.modeling_tools._batch_cache[1]
+    solver.model_parameters = start_model
+    nt0 = solver.nsteps
+    m1 = copy.deepcopy(start_model)
+    m1.C = np.asarray(m1.C).reshape(-1, 1) * 1.03  # every speed up 3 %: dt shrinks
+    solver.model_parameters = m1
+    assert solver.nsteps > nt0
+    aux = {"objective_value": (True, None)}
+    g1 = obj.compute_gradient(shots, m1, aux_info=aux)
+    assert obj.modeling_tools._batch_cache[1] is b0 and b0.max_nt == solver.nsteps
+    f1 = obj.evaluate(shots, m1)
+    fresh = pa.TemporalLeastSquares(solver)
+    g1_ref = fresh.compute_gradient(shots, m1)
+    f1_ref = fresh.evaluate(shots, m1)
+    assert f1 == f1_ref and aux["objective_value"][1] == f1_ref
+    assert np.array_equal(np.asarray(g1.data), np.asarray(g1_ref.data))
+    # and back to the first time axis
+    assert obj.evaluate(shots, start_model) == f0
+    assert obj.modeling_tools._batch_cache[1] is b0
+
+
 @pytest.fixture(scope="module")
 def fwi_runs():
     """Two bands (5 and 8 Hz) of 3 LBFGS iterations each, FP32 and FP64."""
