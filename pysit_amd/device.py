@@ -177,6 +177,48 @@ def checkpoint_levels(nt, per_slot, ckpt, budget, max_levels=4, state=None):
     return None
 
 
+def _beta(s, t):
+    """C(s + t, s): the most steps s snapshots reverse with t sweeps."""
+    from math import comb
+    return comb(s + t, s)
+
+
+def binomial_split(n, s):
+    """Steps to advance before taking the next snapshot when reversing ``n``
+    steps from a stored state with ``s`` free snapshots: the binomial
+    (Revolve) schedule of Griewank (1992) and Griewank & Walther (2000).
+    With S = s + 1 snapshots counting the stored state and the repetition
+    number t, the least with C(S + t, S) >= n, an advance m is optimal when
+    the part after the new snapshot (n - m steps, S - 1 snapshots) needs t
+    sweeps, C(S - 1 + t - 1, S - 1) <= n - m <= C(S - 1 + t, S - 1), and the
+    part before it (m steps, S snapshots) t - 1, C(S + t - 2, S) <= m <=
+    C(S + t - 1, S).  The total is then t n - C(S + t, S + 1) forward steps,
+    the least any schedule needs (``binomial_cost``; the CPU tests check it
+    against a brute-force search over every split)."""
+    if n <= 1:
+        return 0
+    if s <= 0:
+        return n - 1
+    S = s + 1
+    t = 0
+    while _beta(S, t) < n:
+        t += 1
+    lo = max(1, n - _beta(S - 1, t), _beta(S, t - 2) if t >= 2 else 1)
+    hi = min(n - 1, n - _beta(S - 1, t - 1), _beta(S, t - 1))
+    return max(1, min(n - 1, (lo + hi) // 2))
+
+
+def binomial_cost(n, s):
+    """Forward steps the binomial schedule advances to reverse n steps with s
+    free snapshots (the one-step stores of the reverse steps not counted)."""
+    if n <= 1:
+        return 0
+    if s <= 0:
+        return n * (n - 1) // 2
+    m = binomial_split(n, s)
+    return m + binomial_cost(n - m, s - 1) + binomial_cost(m, s)
+
+
 GRAPH_MAX_NODES = 1 << 20
 # FP32 imaging window (slots summed in FP32 before each FP64 flush) and the
 # default graph chunk, a multiple of it so that graph chunks hold whole windows
@@ -233,6 +275,13 @@ class DeviceBatch:
         self.tile_y = N.tile_rows(self.dim, 1 if self.dtype == np.float64 else 0, 1) if self.dim == 3 else 1
         self.zalign = z_align(self.dtype.itemsize)
         self.memory_budget = memory_budget
+        # checkpoint schedule: "auto" stores what fits and nests segment
+        # checkpoints (falling back to the binomial schedule when even four
+        # levels do not fit); "binomial" always reverses from rest with state
+        # snapshots (max_snaps caps their number)
+        self.schedule = "auto"
+        self.max_snaps = None
+        self.snaps = None
         # FP32 adjoints sum the imaging condition in FP32 over windows of
         # `ic_window` slots and in FP64 across windows (0: FP64 every step)
         if ic_window is None:
@@ -767,8 +816,11 @@ class DeviceBatch:
         memory budget (default: 90% of the memory available to this batch:
         free device memory, memory torch holds unused, and this batch's own
         store and checkpoints, which a new run reuses or releases)."""
+        self.snaps = None
         if not keep_dw:
             return self.max_nt
+        if self.schedule not in ("auto", "binomial"):
+            raise ValueError("schedule must be 'auto' or 'binomial'")
         budget = self.memory_budget
         item = self.dtype.itemsize
         n_inj = int(self.adj_inj[2].numel()) - 1 if self.adj_inj is not None else 0
@@ -792,11 +844,16 @@ class DeviceBatch:
         # accumulators
         budget -= state + 12 * self.total_n
         self.levels = [self.max_nt]
+        self.snaps = None
+        if self.schedule == "binomial":
+            return self._plan_binomial(budget, per_slot, ckpt, state)
         if self.max_nt * per_slot <= budget:
             return self.max_nt
         levels = checkpoint_levels(self.max_nt, per_slot, ckpt, budget, state=state)
         if levels is None:
-            raise MemoryError("memory budget too small for the checkpoint schedule")
+            # fewer bytes than the nested schedule needs: state snapshots
+            # alone, reversed step by step (binomial schedule)
+            return self._plan_binomial(budget, per_slot, ckpt, state)
         held = getattr(self, "dw_slots", 0) if getattr(self, "dw", None) is not None else 0
         if (self.memory_budget is None and len(levels) == 1 and held and held == getattr(self, "K", 0)
                 and abs(levels[0] - held) <= max(2, levels[0] // 20)):
@@ -808,16 +865,34 @@ class DeviceBatch:
         self.levels = levels
         return levels[0]
 
+    def _plan_binomial(self, budget, per_slot, ckpt, state):
+        """The binomial schedule: the forward pass only records traces; the
+        adjoint reverses the run from rest with as many state snapshots as
+        fit (a working state and a one-slot dWaveOp store beside them), each
+        reverse step recomputing its own dWaveOp.  Returns K = 1."""
+        snaps = (budget - state - per_slot) // ckpt
+        if snaps < 0:
+            raise MemoryError("memory budget too small for the checkpoint schedule")
+        snaps = int(min(snaps, self.max_nt))
+        if self.max_snaps is not None:
+            snaps = min(snaps, int(self.max_snaps))
+        self.snaps = snaps
+        self.levels = [1]
+        return 1
+
     def _ensure_dw(self, K):
+        # zeroed once per allocation: no step writes the row padding past nz
+        # (or the slots before an uneven shot starts), and sums over a
+        # shot's whole region (the dWaveOp squares) read it
         if self.dw is None or self.dw_slots < K:
             self.dw = None
             self.dw_slots = 0
             try:
-                self.dw = torch.empty((K, self.total_n), dtype=self.tdtype, device=self.dev)
+                self.dw = torch.zeros((K, self.total_n), dtype=self.tdtype, device=self.dev)
             except torch.cuda.OutOfMemoryError:
                 # the released store may still sit in torch's cache
                 torch.cuda.empty_cache()
-                self.dw = torch.empty((K, self.total_n), dtype=self.tdtype, device=self.dev)
+                self.dw = torch.zeros((K, self.total_n), dtype=self.tdtype, device=self.dev)
             self.dw_slots = K
 
     # ----------------------------------------------------------------- forward
@@ -846,6 +921,16 @@ class DeviceBatch:
         self.ckpts = []
         self._dw_seg = None
         self.field_store = self._field_store() if store_field else None
+        if keep_dw and self.snaps is not None and not segment:
+            # binomial schedule: the adjoint reverses the run from rest; the
+            # forward pass only records the traces
+            if store_field:
+                raise ValueError("storing the wavefield needs the whole dWaveOp history in memory")
+            self._run(self._batch(self.fstate, False), 0, 0, self.max_nt, 1, 0)
+            self.have_dw = True
+            return self.traces_in_caller_order()
+        if segment:
+            self.snaps = None
         if keep_dw and K >= self.max_nt:
             self._ensure_dw(self.max_nt)
             b = self._batch(self.fstate, False, dw=self.dw)
@@ -1022,7 +1107,8 @@ class DeviceBatch:
         if self.adj_inj is None:
             raise RuntimeError("the shots carry no adjoint injection entries")
         img_hist = getattr(self, image_store, None) if image else None
-        if image and img_hist is None:
+        recomputed = image_store == "dw" and (self.ckpts or self.snaps is not None)
+        if image and img_hist is None and not recomputed:
             raise RuntimeError(f"no {image_store} history is held")
         res = [None] * self.n_shots
         for pos, orig in enumerate(self.order_idx):
@@ -1054,8 +1140,9 @@ class DeviceBatch:
         # segment) are evaluated before its first step: a step then reads one
         # value per node instead of gathering the node's receiver entries
         n_nodes = int(self.adj_inj[2].numel()) - 1
+        binomial = image and self.snaps is not None
         if n_nodes > 0 and self.precompute_injection:
-            slots = K if (self.ckpts and image) else self.max_nt
+            slots = K if ((self.ckpts or binomial) and image) else self.max_nt
             qb.inj_pre = self._inj_pre_buffer(slots, n_nodes).data_ptr()
             qb.inj_node_shot = self.adj_inj[5].data_ptr()
             qb.n_inj_nodes = n_nodes
@@ -1078,9 +1165,19 @@ class DeviceBatch:
             qb.vol_m = vm.data_ptr()
             qb.vol_dw = vd.data_ptr()
             self._vol_keep = (vm, vd)
+        if binomial and second:
+            raise RuntimeError("the second-order terms need every step of the forward stored")
         if not image:
             qb.dw = None
             self._run(qb, 2, 0, self.max_nt, -1, 0, record=False)
+        elif binomial:
+            self._ensure_dw(1)
+            qb.dw = self.dw.data_ptr()
+            work = self._new_state()
+            fb = self._batch(work, False, dw=self.dw)
+            wb = self._batch(work, False)
+            self.binomial_advances = 0
+            self._treeverse(0, self.max_nt, self.snaps, None, work, fb, wb, qb)
         elif not self.ckpts:
             qb.dw = img_hist.data_ptr()
             self._dw_squares(0, self.max_nt, 0)
@@ -1096,6 +1193,40 @@ class DeviceBatch:
                 j0 = si * K
                 self._reverse(j0, min(self.max_nt, j0 + K), self.ckpts[si], 1, levels, work, fb, wb, qb)
         return self.ic
+
+    def _treeverse(self, a, b, snaps, ckpt, work, fb, wb, qb):
+        """The adjoint over slots [a, b) with the binomial schedule
+        (``binomial_split``): the state at slot a is ``ckpt`` (None: rest, at
+        slot 0) and ``snaps`` more snapshots may be taken.  Each reverse step
+        restores or advances the state to its slot, recomputes that slot's
+        dWaveOp into the one-slot store and runs the adjoint step on it."""
+        while b - a > 0:
+            if b - a == 1 or snaps == 0:
+                k = b - 1
+                self._restore_or_rest(ckpt, work, a)
+                if k > a:
+                    self._run(wb, 0, a, k, 1, a, record=False)
+                    self.binomial_advances += k - a
+                self._run(fb, 1, k, k + 1, 1, k, record=False)
+                self._dw_squares(k, k + 1, k)
+                self._run(qb, 2, k, k + 1, -1, k, record=False)
+                b = k
+                continue
+            m = a + binomial_split(b - a, snaps)
+            self._restore_or_rest(ckpt, work, a)
+            self._run(wb, 0, a, m, 1, a, record=False)
+            self.binomial_advances += m - a
+            snap = self._ckpt_take(work, m)
+            self._treeverse(m, b, snaps - 1, snap, work, fb, wb, qb)
+            del snap
+            b = m
+
+    def _restore_or_rest(self, ckpt, work, slot):
+        if ckpt is None:
+            for f in work.values():
+                f.zero_()
+        else:
+            self._ckpt_restore(ckpt, work, slot)
 
     def _reverse(self, j0, j1, ckpt, lvl, levels, work, fb, wb, qb):
         """The adjoint over slots [j0, j1) from the state checkpoint taken at

@@ -156,6 +156,60 @@ def test_nested_checkpoint_levels_equal_store_all(gpu, budget_kib):
     assert np.array_equal(ic1, ic2)
 
 
+@pytest.mark.parametrize("snaps", [0, 2, 5])
+def test_binomial_schedule_equals_store_all(gpu, snaps):
+    """The binomial (Revolve) schedule reverses the run from rest with a few
+    state snapshots, recomputing each step's dWaveOp: the same bits as
+    storing every step, uneven shots included (each shot's level at a
+    snapshot slot), and exactly the forward advances binomial_cost counts;
+    the pseudo-Hessian diagonal sums the recomputed dWaveOp's squares.""" 
+    from pysit_amd.device import DeviceBatch, binomial_cost
+    base = Synthetic((60, 50), ((6, 6), (5, 5)), (0.01, 0.01), 90, 8, seed=3)
+    probs = []
+    for s, (w0, w1, nt) in enumerate([(0, 40, 90), (10, 47, 61), (5, 30, 77)]):
+        window = [(6 - w0, w0, w1 - 1), (5, 0, base.model.shape[1] - 1)]
+        probs.append(Synthetic((w1 - w0 + 12, 50), ((6, 6), (5, 5)), (0.01, 0.01), nt, 8, seed=30 + s,
+                               window=window, model=base.model, dt=base.dt))
+    rng = np.random.default_rng(4)
+    res = [rng.standard_normal((p.nt, p.n_rec)) for p in probs]
+    out = []
+    for schedule in ("auto", "binomial"):
+        b = DeviceBatch([p.geometry() for p in probs], base.model, base.deltas, base.dt, 8, dtype=np.float64)
+        b.schedule = schedule
+        b.max_snaps = snaps
+        tr = [t.cpu().numpy().copy() for t in b.forward(keep_dw=True)]
+        b.migrate(res, dw_squares=True)
+        out.append((tr, b.ic.cpu().numpy().copy(), b.dw_sq.cpu().numpy().copy()))
+        if schedule == "binomial":
+            assert b.snaps == snaps and b.binomial_advances == binomial_cost(b.max_nt, snaps)
+    assert all(np.array_equal(a, c) for a, c in zip(out[0][0], out[1][0]))
+    assert np.array_equal(out[0][1], out[1][1])
+    # the squares are summed slot by slot in reverse order there
+    assert np.linalg.norm(out[0][2] - out[1][2]) <= 1e-13 * np.linalg.norm(out[0][2])
+
+
+def test_binomial_schedule_is_the_fallback_below_the_nested_one(gpu):
+    """A budget below every nested schedule (four levels) but above a few
+    state snapshots plans the binomial schedule instead of raising; FP32."""
+    from pysit_amd.device import DeviceBatch
+    p = Synthetic((50, 70), ((5, 5), (4, 6)), (0.01, 0.01), 240, 8, seed=11)
+    res = [np.random.default_rng(2).standard_normal((p.nt, p.n_rec))]
+    b1 = _batch([p], np.float32)
+    b1.forward(keep_dw=True)
+    ic1 = b1.migrate(res).cpu().numpy()
+    probe = _batch([p], np.float32)
+    state, ckpt = probe.state_bytes(), probe.ckpt_bytes()
+    n_inj = int(probe.adj_inj[2].numel()) - 1
+    per_slot = (probe.total_n + n_inj) * 4
+    budget = 2 * state + 12 * probe.total_n + per_slot + 3 * ckpt + 100
+    b2 = _batch([p], np.float32, memory_budget=budget)
+    b2.forward(keep_dw=True)
+    assert b2.snaps == 3 and b2.ckpts == []
+    ic2 = b2.migrate(res).cpu().numpy()
+    # FP32: the window sums flush after every one-step run here
+    assert np.linalg.norm(ic2 - ic1) <= 1e-6 * np.linalg.norm(ic1)
+
+
 def test_batch_uneven_shots_checkpointed_equal_store_all(gpu):
     # uneven shots sit at different step parities at a checkpoint slot: each
     # shot's current level is saved and restored

@@ -345,3 +345,31 @@ def test_batch_limits_follow_the_kernels_addressable_region():
     k2d = device.shots_per_batch((1041, 341), 8)
     assert k2d * n2 * 56 <= lim < (k2d + 1) * n2 * 56
     assert device.shots_per_batch((2000, 2000, 2000), 4) == 1
+
+
+def test_binomial_schedule_is_optimal():
+    """The binomial checkpoint schedule's split (device.binomial_split)
+    reverses n steps with s free snapshots in the fewest forward steps: equal
+    to a brute-force search over every split, and to the closed form
+    t n - C(S + t, S + 1) with S = s + 1 and t the least with C(S + t, S) >= n
+    (Griewank and Walther)."""
+    from functools import lru_cache
+    from math import comb
+    from pysit_amd.device import binomial_cost
+
+    @lru_cache(None)
+    def brute(n, s):
+        if n <= 1:
+            return 0
+        if s == 0:
+            return n * (n - 1) // 2
+        return min(m + brute(n - m, s - 1) + brute(m, s) for m in range(1, n))
+
+    for s in range(0, 6):
+        for n in range(1, 120):
+            assert binomial_cost(n, s) == brute(n, s), (n, s)
+            if s and n > 1:
+                S, t = s + 1, 0
+                while comb(S + t, S) < n:
+                    t += 1
+                assert binomial_cost(n, s) == t * n - comb(S + t, S + 1)
