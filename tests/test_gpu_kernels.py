@@ -210,6 +210,48 @@ def test_binomial_schedule_is_the_fallback_below_the_nested_one(gpu):
     assert np.linalg.norm(ic2 - ic1) <= 1e-6 * np.linalg.norm(ic1)
 
 
+@pytest.mark.parametrize("budget_kib", [6000, 3000])
+def test_retimed_batch_equals_a_fresh_one(gpu, budget_kib):
+    """DeviceBatch.retime keeps the tables and takes a new time axis (new
+    step counts and amplitudes at another dt); with segment checkpoints
+    (6000 KiB) or binomial snapshots (3000 KiB) and uneven shots the traces
+    and images equal a fresh batch's bit for bit, before and after going
+    back to the first axis."""
+    def shots(nts, dt=None):
+        out = []
+        for k, nt in enumerate(nts):
+            out.append(Synthetic((50, 70), ((5, 5), (4, 6)), (0.01, 0.01), nt, 8, seed=11 + k,
+                                 model=shots.model, dt=dt))
+        return out
+    shots.model = Synthetic((50, 70), ((5, 5), (4, 6)), (0.01, 0.01), 10, 8, seed=11).model
+    a = shots((240, 200))
+    c = shots((220, 180), dt=a[0].dt * 1.09)
+    rng = np.random.default_rng(3)
+    res_a = [rng.standard_normal((p.nt, p.n_rec)) for p in a]
+    res_c = [rng.standard_normal((p.nt, p.n_rec)) for p in c]
+    budget = budget_kib * 1024
+
+    def run(b, probs, res):
+        tr = [t.cpu().numpy().copy() for t in b.forward(keep_dw=True)]
+        return tr, b.migrate(res).cpu().numpy().copy(), list(b.levels)
+
+    b = _batch(a, np.float64, memory_budget=budget)
+    first = run(b, a, res_a)
+    b.retime([p.geometry() for p in c], c[0].dt)
+    got = run(b, c, res_c)
+    want = run(_batch(c, np.float64, memory_budget=budget), c, res_c)
+    print("levels", first[2], got[2], want[2], "snaps", b.snaps)
+    assert got[2] == want[2] and (b.snaps is None) == (budget_kib == 6000)
+    assert all(np.array_equal(x, y) for x, y in zip(got[0], want[0]))
+    assert np.array_equal(got[1], want[1])
+    b.retime([p.geometry() for p in a], a[0].dt)
+    again = run(b, a, res_a)
+    assert all(np.array_equal(x, y) for x, y in zip(again[0], first[0]))
+    assert np.array_equal(again[1], first[1])
+    with pytest.raises(ValueError):
+        b.retime([p.geometry() for p in a[::-1]], a[0].dt)
+
+
 def test_batch_uneven_shots_checkpointed_equal_store_all(gpu):
     # uneven shots sit at different step parities at a checkpoint slot: each
     # shot's current level is saved and restored
