@@ -11,9 +11,9 @@ numpy, the small host-side pieces of the reference path that the checks need:
 
 * ``interp_zero_fill`` -- ``scipy.interpolate.interp1d(kind='linear',
   bounds_error=False, fill_value=0.0)`` as used by
-  ``pysit/core/receivers.py:75-105`` (parity unpinned against the library
-  itself: scipy is a third-party dependency; pinned by closed forms in the
-  tests -- exact on piecewise-linear data, zero outside the range).
+  ``pysit/core/receivers.py:75-105`` (pinned in the tests against scipy's
+  own interp1d and by closed forms: exact on piecewise-linear data, zero
+  outside the range).
 * ``misfit`` -- ``0.5 * dt * sum ||r||^2``
   (``pysit/objective_functions/temporal_least_squares.py:67-82``).
 """

@@ -208,6 +208,22 @@ def test_interp_zero_fill_closed_forms():
     assert np.allclose(out, want, atol=1e-14)
 
 
+def test_interp_zero_fill_matches_scipy():
+    """The oracle's restatement of the receivers' interpolation equals
+    scipy's interp1d(kind='linear', bounds_error=False, fill_value=0) -- the
+    call receivers.py:75-105 makes -- on irregular data, a finer and a
+    coarser time axis and points outside the range."""
+    from scipy.interpolate import interp1d
+    rng = np.random.default_rng(12)
+    ts = np.cumsum(rng.uniform(0.5, 1.5, 200)) * 1e-3
+    data = rng.standard_normal((200, 7))
+    for tn in (np.linspace(-0.01, ts[-1] + 0.01, 731), np.linspace(ts[0], ts[-1], 57),
+               np.sort(rng.uniform(-0.02, ts[-1] + 0.02, 300))):
+        want = interp1d(ts, data, axis=0, kind="linear", bounds_error=False, fill_value=0.0)(tn)
+        got = O.interp_zero_fill(ts, data, tn)
+        assert np.max(np.abs(got - want)) <= 4e-16 * np.max(np.abs(want))
+
+
 @pytest.mark.parametrize("tag", ["c1", "c3d"])
 def test_oracle_pml_composition_matches_reference_python(tag):
     """With the PML on, the oracle's forward loop (traces, dWaveOp with the
