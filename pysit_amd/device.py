@@ -1099,9 +1099,12 @@ class DeviceBatch:
         source vol_m * history_k (vol_m: a flat batch array, see
         padded_operand; history: a (max_nt, total_n) store)."""
         second = image_store != "dw" or store_adjoint_dw or volume is not None
-        if second and (self.imaging_period != 1 or (image and self.ckpts)):
+        binomial = image and self.snaps is not None
+        if second and (self.imaging_period != 1 or (image and (self.ckpts or binomial))):
             raise RuntimeError("an adjoint with second-order terms needs imaging_period 1 and, when it images, "
                                "every step of the forward stored")
+        if store_field and image and (self.ckpts or binomial):
+            raise ValueError("storing the adjoint field needs the whole dWaveOp history (no checkpoints)")
         if image and not getattr(self, "have_dw", False):
             raise RuntimeError("migrate needs a forward(keep_dw=True) first")
         if self.adj_inj is None:
@@ -1140,7 +1143,6 @@ class DeviceBatch:
         # segment) are evaluated before its first step: a step then reads one
         # value per node instead of gathering the node's receiver entries
         n_nodes = int(self.adj_inj[2].numel()) - 1
-        binomial = image and self.snaps is not None
         if n_nodes > 0 and self.precompute_injection:
             slots = K if ((self.ckpts or binomial) and image) else self.max_nt
             qb.inj_pre = self._inj_pre_buffer(slots, n_nodes).data_ptr()
@@ -1151,8 +1153,6 @@ class DeviceBatch:
         if store_field:
             self.adjoint_store = self._field_store()
             qb.fstore = self.adjoint_store.data_ptr()
-            if self.ckpts and image:
-                raise ValueError("storing the adjoint field needs seg_base 0 (no checkpoints)")
         if store_adjoint_dw:
             self.dw_adj = None
             self.dw_adj = torch.zeros((self.max_nt, self.total_n), dtype=self.tdtype, device=self.dev)
@@ -1165,8 +1165,6 @@ class DeviceBatch:
             qb.vol_m = vm.data_ptr()
             qb.vol_dw = vd.data_ptr()
             self._vol_keep = (vm, vd)
-        if binomial and second:
-            raise RuntimeError("the second-order terms need every step of the forward stored")
         if not image:
             qb.dw = None
             self._run(qb, 2, 0, self.max_nt, -1, 0, record=False)
