@@ -37,6 +37,8 @@ Cases (closed-form inputs, no random generator in the fixture's recipe):
   (0.02 sin(7x + 1) cos(5z) cos(3y) in 3D), and the objective history and
   final model of three LBFGS iterations (memory 3, backtracking line search,
   optimization.py:312-501, lbfgs.py:39-124).
+* ``c1o8_*``, ``c3do8_*`` -- the same two cases at spatial order 8: traces,
+  dWaveOp, migrate_shot, objective and gradient.
 * ``c3d_*`` -- a 3D layered block (x, y, z [0, 0.3], 16^3 nodes, PML(0.08,
   60) on all sides, order 4, T = 0.25, one 12 Hz Ricker, a 4 x 4 receiver
   patch), the same quantities but LBFGS: traces every 2nd step, dWaveOp at
@@ -161,7 +163,7 @@ def born_direction(m):
     return (0.02 * np.sin(7.0 * X + 1.0) * np.cos(5.0 * Z)).reshape(-1, 1)
 
 
-def c1(ps):
+def c1(ps, order=4):
     p = ps.PML(0.1, 100)
     d = ps.RectangularDomain((0.0, 0.9, p, p), (0.0, 0.7, p, p))
     m = ps.CartesianMesh(d, 91, 71)
@@ -170,12 +172,14 @@ def c1(ps):
     C0 = (1.0 + 0.3 * np.exp(-((Z - 0.45) / 0.02) ** 2)).reshape(-1, 1)
     shots = ps.equispaced_acquisition(m, ps.RickerWavelet(10.0), sources=1, source_depth=0.05,
                                       receivers="max", receiver_depth=0.05)
-    solver = ps.ConstantDensityAcousticWave(m, spatial_accuracy_order=4, trange=(0.0, 1.5),
+    solver = ps.ConstantDensityAcousticWave(m, spatial_accuracy_order=order, trange=(0.0, 1.5),
                                             kernel_implementation="cpp")
+    if order != 4:
+        return run_case(ps, m, shots, solver, C, C0, 4, [0.4, 0.9])
     return run_case(ps, m, shots, solver, C, C0, 4, [0.4, 0.9], second_order=True, lbfgs=True)
 
 
-def block3d(ps):
+def block3d(ps, order=4):
     p = ps.PML(0.08, 60)
     d = ps.RectangularDomain((0.0, 0.3, p, p), (0.0, 0.3, p, p), (0.0, 0.3, p, p))
     m = ps.CartesianMesh(d, 16, 16, 16)
@@ -187,15 +191,16 @@ def block3d(ps):
     src = ps.PointSource(m, (0.15, 0.15, 0.04), ps.RickerWavelet(12.0))
     recs = ps.ReceiverSet(m, [ps.PointReceiver(m, (a, b, 0.04)) for a in rec for b in rec])
     shots = [ps.Shot(src, recs)]
-    solver = ps.ConstantDensityAcousticWave(m, spatial_accuracy_order=4, trange=(0.0, 0.25),
+    solver = ps.ConstantDensityAcousticWave(m, spatial_accuracy_order=order, trange=(0.0, 0.25),
                                             kernel_implementation="cpp")
-    return run_case(ps, m, shots, solver, C, C0, 2, [0.7], second_order=True)
+    return run_case(ps, m, shots, solver, C, C0, 2, [0.7], second_order=order == 4)
 
 
 def main():
     ps = import_reference()
     out = {}
-    for tag, fn in (("c1", c1), ("c3d", block3d)):
+    cases = (("c1", c1), ("c3d", block3d), ("c1o8", lambda ps: c1(ps, 8)), ("c3do8", lambda ps: block3d(ps, 8)))
+    for tag, fn in cases:
         res = fn(ps)
         print(tag, {k: np.shape(v) for k, v in res.items()}, flush=True)
         out.update({f"{tag}_{k}": v for k, v in res.items()})

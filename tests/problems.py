@@ -214,10 +214,13 @@ def truncated_geometry(g, nt):
 
 
 def pml_golden_case(tag, precision="double"):
-    """The two cases of tests/golden/pml_golden.npz built with this package's
-    API (tests/golden/make_pml_golden.py builds them with PySIT's): returns
-    (mesh, shots, solver, true C, start C), C as (dof, 1) arrays."""
+    """The cases of tests/golden/pml_golden.npz built with this package's
+    API (tests/golden/make_pml_golden.py builds them with PySIT's): "c1" and
+    "c3d" at spatial order 4, "c1o8" and "c3do8" at order 8; returns (mesh,
+    shots, solver, true C, start C), C as (dof, 1) arrays."""
     import pysit_amd as pa
+    order = 8 if tag.endswith("o8") else 4
+    tag = tag[:-2] if tag.endswith("o8") else tag
     if tag == "c1":
         p = pa.PML(0.1, 100)
         d = pa.RectangularDomain((0.0, 0.9, p, p), (0.0, 0.7, p, p))
@@ -241,7 +244,7 @@ def pml_golden_case(tag, precision="double"):
         recs = pa.ReceiverSet(m, [pa.PointReceiver(m, (a, b, 0.04)) for a in rec for b in rec])
         shots = [pa.Shot(src, recs)]
         tr = (0.0, 0.25)
-    solver = pa.ConstantDensityAcousticWave(m, spatial_accuracy_order=4, trange=tr, kernel_implementation="hip",
+    solver = pa.ConstantDensityAcousticWave(m, spatial_accuracy_order=order, trange=tr, kernel_implementation="hip",
                                             precision=precision)
     return m, shots, solver, C, C0
 

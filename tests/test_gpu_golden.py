@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pml_golden.npz")
 
 
-@pytest.mark.parametrize("tag", ["c1", "c3d"])
+@pytest.mark.parametrize("tag", ["c1", "c3d", "c1o8", "c3do8"])
 def test_device_path_matches_reference_python_with_pml(gpu, tag):
     import pysit_amd as pa
     g = np.load(GOLDEN)

@@ -224,7 +224,7 @@ def test_interp_zero_fill_matches_scipy():
         assert np.max(np.abs(got - want)) <= 4e-16 * np.max(np.abs(want))
 
 
-@pytest.mark.parametrize("tag", ["c1", "c3d"])
+@pytest.mark.parametrize("tag", ["c1", "c3d", "c1o8", "c3do8"])
 def test_oracle_pml_composition_matches_reference_python(tag):
     """With the PML on, the oracle's forward loop (traces, dWaveOp with the
     mesh profile and, in 3D, the psi term), its migrate with the fold, and
