@@ -373,3 +373,40 @@ def test_binomial_schedule_is_optimal():
                 while comb(S + t, S) < n:
                     t += 1
                 assert binomial_cost(n, s) == t * n - comb(S + t, S + 1)
+
+
+def test_psi_tile_flag_matches_a_node_by_node_count():
+    """3D tile flags: bit2 (no psi) is set exactly for the tiles in whose
+    reach (owned nodes +- order/2) no node lies in two or more PML bands,
+    counted node by node; bit0 (PML path) whenever a node of the reach lies
+    in a PML band or on the grid's edge."""
+    from types import SimpleNamespace
+    from pysit_amd.device import DeviceBatch, z_tiles_2d
+    shape = (70, 41, 150)
+    widths = ((5, 6), (4, 0), (7, 3))
+    kpml = [(np.full(l, 10.0), np.full(r, 10.0)) for l, r in widths]
+    fake = object.__new__(DeviceBatch)
+    fake.R, fake.dim, fake.tile_x, fake.tile_y, fake.zalign = 4, 3, 16, 8, 32
+    (ntx, nty, ntz), flags = fake._tiles_of(SimpleNamespace(shape=shape, kpml=kpml))
+    band = [np.zeros(n, dtype=int) for n in shape]
+    edge = [np.zeros(n, dtype=bool) for n in shape]
+    for a, (n, (l, r)) in enumerate(zip(shape, widths)):
+        band[a][:l] = 1
+        if r:
+            band[a][n - r:] = 1
+        edge[a][[0, n - 1]] = True
+    count = band[0][:, None, None] + band[1][None, :, None] + band[2][None, None, :]
+    onedge = edge[0][:, None, None] | edge[1][None, :, None] | edge[2][None, None, :]
+    zt = z_tiles_2d(shape[2], 32)
+    R = 4
+    for tx in range(ntx):
+        xs = slice(max(0, tx * 16 - R), min(shape[0], (tx + 1) * 16 + R))
+        for ty in range(nty):
+            ys = slice(max(0, ty * 8 - R), min(shape[1], (ty + 1) * 8 + R))
+            for tz, (z0, zhi) in enumerate(zt):
+                zs = slice(max(0, z0 - R), min(shape[2], zhi + R))
+                f = int(flags[(tx * nty + ty) * ntz + tz])
+                c = count[xs, ys, zs]
+                assert bool(f & 4) == (c.max() <= 1), (tx, ty, tz)
+                assert bool(f & 1) == bool(c.max() >= 1 or onedge[xs, ys, zs].any()), (tx, ty, tz)
+    assert (flags & 4).sum() and ((flags & 1) & ~(flags >> 2) & 1).sum()
