@@ -279,13 +279,13 @@ def write_traffic(fetch_csv, write_csv, workload="c2"):
                      "(bench.py --pmc-run, 40 slots)"),
         "shots": shots, "grid_padded": list(shape), "tile_x": tx,
         "commands": ([] if workload == "c4" else [
-            "PYSIT_AMD_TUNE_CACHE=gpurun_out/tune.json python3 bench.py  (the launch configuration chosen)"]) + [
-            "PYSIT_AMD_TUNE_CACHE=gpurun_out/tune.json rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmcf -o run "
+            "PYSIT_AMD_TUNE_CACHE=prof/tune.json python3 bench.py  (the launch configuration chosen)"]) + [
+            "PYSIT_AMD_TUNE_CACHE=prof/tune.json rocprofv3 --pmc FETCH_SIZE -d prof/pmcf -o run "
             f"--output-format csv -- python3 bench.py{wflag} --pmc-run --steps 1 --warmup 0",
-            "PYSIT_AMD_TUNE_CACHE=gpurun_out/tune.json rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmcw -o run "
+            "PYSIT_AMD_TUNE_CACHE=prof/tune.json rocprofv3 --pmc WRITE_SIZE -d prof/pmcw -o run "
             f"--output-format csv -- python3 bench.py{wflag} --pmc-run --steps 1 --warmup 0",
-            f"python3 bench.py{wflag} --write-traffic gpurun_out/pmcf/run_counter_collection.csv "
-            "gpurun_out/pmcw/run_counter_collection.csv"],
+            f"python3 bench.py{wflag} --write-traffic prof/pmcf/run_counter_collection.csv "
+            "prof/pmcw/run_counter_collection.csv"],
         "calibration": {"kernel": "torch.add(x, 1.0, out=y) over 2^26 float32 (reads and writes 262144 KiB)",
                         "fetch_kib": f["cal"], "write_kib": w["cal"]},
         "fetch_factor": round(262144.0 / f["cal"], 4), "write_factor": round(262144.0 / w["cal"], 4),
@@ -567,8 +567,8 @@ def main():
         write_traffic(*args.write_traffic, workload=args.workload)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (shots shard across GPUs "
-              f"with one process per GPU, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported here (more GPUs would each run "
+              f"a share of the shots in a process of their own and sum the images and misfits)", file=sys.stderr)
         sys.exit(2)
     import torch
     import pysit_amd as pa

@@ -2,10 +2,9 @@
  * cda_oracle.c -- CPU oracle for PySIT's time-domain constant-density acoustic
  * (CDA) leapfrog + PML solver and its adjoint-state imaging condition.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the smoke()
- * entry and the cpu_baseline leg of bench.py load it to compare the HIP
- * library against.  The product package (pysit_amd) never links, loads or
- * calls it.
+ * TEST INFRASTRUCTURE ONLY.  tests/, the smoke() entry and the cpu_baseline
+ * leg of bench.py load it as the CPU result the HIP library is compared
+ * against.  The product package (pysit_amd) never links, loads or calls it.
  *
  * What it restates (reference paths relative to the PySIT source tree):
  *   orc_step2d_{d,f}   pysit/solvers/constant_density_acoustic/time/scalar/

@@ -1,8 +1,8 @@
 """Python side of the CPU oracle (test infrastructure only).
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` import this module, and only as the checker.  The product package
-``pysit_amd`` never imports it.
+``bench.py`` import this module, and only as the CPU result the HIP library is
+compared against.  The product package ``pysit_amd`` never imports it.
 
 It loads ``oracle/liborc.so`` (the C restatement in ``cda_oracle.c``) and, when
 it was built here from the PySIT sources, ``oracle/_ref/libcda_ref.so`` (the
