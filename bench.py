@@ -52,7 +52,7 @@ def parse():
     p.add_argument("--tile-x", type=int, default=None)
     p.add_argument("--memory-budget-gb", type=float, default=None)
     p.add_argument("--profile-run", action="store_true", help="minimal run for rocprofv3 (no plain, no CPU)")
-    p.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2",
+    p.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5"], default="c2",
                    help="c2: the headline 16-shot gradient; c3: 64 shots of uneven aperture and length; "
                         "c4: the 3D 256^3 8-shot checkpointed gradient (use --steps 1 --warmup 0); "
                         "c5: the multi-band LBFGS FWI loop (FP32, then FP64 for comparison)")
@@ -190,7 +190,7 @@ def cpu_gradient(solver, shot, seconds):
     return {"value": 2.0 * n * op.nsteps / t / 1e9, "unit": "Gpt-updates/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
             "kind": "port", "seconds": t,
             "sample": f"one shot's FP64 gradient on the {'x'.join(map(str, op.shape))} padded grid, "
-                      f"{op.nsteps} of its {full} steps, order 8, the oracle's forward + migrate loops "
+                      f"{op.nsteps} of its {full} steps, order {op.order}, the oracle's forward + migrate loops "
                       f"(-O3 -fopenmp -ffast-math, node loops over {os.environ['OMP_NUM_THREADS']} threads; "
                       f"four propagations, the rate counts two)"}
 
@@ -301,6 +301,65 @@ def write_traffic(fetch_csv, write_csv, workload="c2"):
         json.dump(doc, fh, indent=1)
         fh.write("\n")
     print(json.dumps(doc))
+
+
+def run_c1(args):
+    """The reference's own CPU case: the 2D horizontal reflector (91 x 71,
+    dx 0.01, order 4, T = 1.5, PML 0.1, one Ricker source, 91 receivers) in
+    FP64 through PySIT's API on the device: forward_model with
+    simdata and dWaveOp, then migrate_shot of the residual against the true
+    model's data, per step of the bench.  Beside it, the same gradient on the
+    host through the oracle's loops (FP64, all host threads), and the device
+    results against it.  Prints one JSON line."""
+    import torch
+    import pysit_amd as pa
+    from pysit_amd.workloads import horizontal_reflector_2d
+    from tests.problems import oracle_from_solver
+    m, shots, solver, model = horizontal_reflector_2d()
+    pa.generate_seismic_data(shots, solver, model)
+    X, Z = m.mesh_coords()
+    m0 = solver.ModelParameters(m, {"C": (1.0 + 0.3 * np.exp(-((Z - 0.45) / 0.02) ** 2)).reshape(-1, 1)})
+    tm = pa.TemporalModeling(solver)
+    shot = shots[0]
+
+    def gradient():
+        rv = tm.forward_model(shot, m0, 1, return_parameters=["simdata", "dWaveOp"])
+        r = shot.receivers.interpolate_data(solver.ts()) - rv["simdata"]
+        return rv["simdata"], r, tm.migrate_shot(shot, m0, r, 1, dWaveOp=rv["dWaveOp"])
+
+    for _ in range(args.warmup):
+        gradient()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim, r, g = gradient()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / args.steps
+    shape = tuple(m.shape(include_bc=True, as_grid=True))
+    n, nt = int(np.prod(shape)), solver.nsteps
+    solver.model_parameters = m0
+    op = oracle_from_solver(solver, shot)
+    ref = op.forward()
+    grad_ref = op.migrate(r)
+
+    def rel(a, b):
+        return float(np.linalg.norm(np.asarray(a).reshape(-1) - np.asarray(b).reshape(-1)) / np.linalg.norm(b))
+    result = {
+        "metric": "C1 forward_model + migrate_shot gradient Gpt-updates/s (FP64, PySIT API, one shot)",
+        "value": 2.0 * n * nt / t / 1e9, "unit": "Gpt-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": t * 1e3, "higher_is_better": True, "dtype": "f64",
+        "data": "the reference's horizontal-reflector case (closed-form model), observed data generated in it; "
+                "gradient at the reflector of 1.3",
+        "config": {"workload": f"2D horizontal reflector, 91x71 ({shape[0]}x{shape[1]} padded), order 4, PML 0.1, "
+                               f"T 1.5 (nt {nt} at the start model's dt), FP64",
+                   "nt": nt, "grid_padded": list(shape)},
+        "vs_oracle": {"traces_rel_l2": rel(sim, ref["simdata"]), "gradient_rel_l2": rel(g.data, grad_ref)},
+    }
+    if not args.no_cpu:
+        cpu = cpu_gradient(solver, shot, 20.0)
+        result["cpu_baseline"] = cpu
+        result["vs_cpu"] = result["value"] / cpu["value"]
+    print(json.dumps(result), flush=True)
 
 
 def run_uneven(args):
@@ -574,6 +633,9 @@ def main():
     import pysit_amd as pa
     from pysit_amd.torch_fn import ShotProblem, ShotTraces
     from pysit_amd.workloads import layered_2d
+    if args.workload == "c1":
+        run_c1(args)
+        return
     if args.workload == "c3":
         run_uneven(args)
         return
