@@ -266,6 +266,23 @@ int pysit_amd_batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int 
                         int direction, int seg_base, const int *active_tiles,
                         const int *active_recs, void *stream);
 
+/* pysit_amd_batch_run as one persistent launch (2D, modes 0, 1 and 2, no
+ * second-order terms).  The reference steps its time loop from Python, one
+ * step call per time level (temporal_modeling.py:120-142, :313-350); here a
+ * cooperative launch runs every step of the call: each block takes its
+ * share of every step's tile and receiver blocks and all blocks meet at a
+ * device-wide barrier between steps (and around the FP32 window's flushes).
+ * The results equal pysit_amd_batch_run's bit for bit.  On MI355X the
+ * barrier (2-4 us at 8-24 blocks) costs more than the launch boundary it
+ * replaces, so the graph loop of pysit_amd_plan_run stays the faster
+ * choice for small batches.  The call allocates a
+ * small device buffer on the stream (the per-step counts and the barrier)
+ * and waits for the launch to end; a barrier that waits for seconds ends the
+ * launch with PYSIT_AMD_EDEVICE. */
+int pysit_amd_batch_run_persistent(const pysit_amd_batch *b, int mode, int slot_begin, int slot_end,
+                                   int direction, int seg_base, const int *active_tiles,
+                                   const int *active_recs, void *stream);
+
 /* pysit_amd_batch_run without a caller workspace.  No counterpart in the
  * reference: this entry point is for callers that cannot allocate device
  * memory (code under hipGraph capture, tight serving loops, harnesses that

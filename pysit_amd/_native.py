@@ -74,6 +74,7 @@ SIGNATURES = {
     "pysit_amd_cda_step3d_f64": _STEP3D,
     "pysit_amd_batch_run": [ctypes.POINTER(Batch), _i, _i, _i, _i, _i, _vp, _vp, _vp],
     "pysit_amd_batch_run_static": [ctypes.POINTER(Batch), _i, _i, _i, _i, _i, _vp, _vp, _vp],
+    "pysit_amd_batch_run_persistent": [ctypes.POINTER(Batch), _i, _i, _i, _i, _i, _vp, _vp, _vp],
     "pysit_amd_fold": [_i] * 10 + [_vp, _d, _vp, _i, _i, _i, _vp],
     "pysit_amd_slot_square_sum": [_vp, _i, _ll, _i, _i, _i, _ll, _vp, _vp],
     "pysit_amd_plan_create": [_i, _i, ctypes.POINTER(_vp)],

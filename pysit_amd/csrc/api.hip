@@ -17,6 +17,9 @@ int run2d(const pysit_amd_batch &, int, int, int, int, int, const int *, const i
 template <typename T>
 int run3d(const pysit_amd_batch &, int, int, int, int, int, const int *, const int *, pysit_amd_plan *,
           hipStream_t);
+template <typename T>
+int run2d_persistent(const pysit_amd_batch &, int, int, int, int, int, const int *, const int *, long long,
+                     hipStream_t);
 
 int tile_rows_2d(int dtype);
 bool tile_rows_2d_supported(int tx);
@@ -291,7 +294,7 @@ namespace pa {
 // pysit_amd_batch_run's checks and dispatch, with an optional plan
 static int batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int slot_end, int direction,
                      int seg_base, const int *active_tiles, const int *active_recs, pysit_amd_plan *plan,
-                     void *stream)
+                     void *stream, long long persistent_spins = 0)
 {
     PA_CHECK(b != nullptr, "null batch");
     const pysit_amd_batch &B = *b;
@@ -343,6 +346,13 @@ static int batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int slo
     if (B.inj_pre) {
         const int rc = inj_precompute(B, slot_begin, slot_end, seg_base, st);
         if (rc) return rc;
+    }
+    if (persistent_spins > 0) {
+        PA_CHECK(B.dim == 2, "the persistent loop is built for 2D batches");
+        return B.dtype ? run2d_persistent<double>(B, mode, slot_begin, slot_end, direction, seg_base, active_tiles,
+                                                  active_recs, persistent_spins, st)
+                       : run2d_persistent<float>(B, mode, slot_begin, slot_end, direction, seg_base, active_tiles,
+                                                 active_recs, persistent_spins, st);
     }
     if (B.dim == 2)
         return B.dtype ? run2d<double>(B, mode, slot_begin, slot_end, direction, seg_base,
@@ -408,6 +418,17 @@ int pysit_amd_batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int 
     clear_error();
     return batch_run(b, mode, slot_begin, slot_end, direction, seg_base, active_tiles, active_recs, nullptr,
                      stream);
+}
+
+int pysit_amd_batch_run_persistent(const pysit_amd_batch *b, int mode, int slot_begin, int slot_end,
+                                   int direction, int seg_base, const int *active_tiles,
+                                   const int *active_recs, void *stream)
+{
+    clear_error();
+    // a barrier wait longer than 2^24 polls of about 0.1 us (seconds, where a
+    // step takes microseconds) ends the launch with an error
+    return batch_run(b, mode, slot_begin, slot_end, direction, seg_base, active_tiles, active_recs, nullptr,
+                     stream, 1LL << 24);
 }
 
 int pysit_amd_batch_run_static(const pysit_amd_batch *b, int mode, int slot_begin, int slot_end,

@@ -40,7 +40,9 @@
 // No MFMA: there is no dense contraction; the step is bound by HBM bandwidth.
 #include "common.h"
 
+#include <algorithm>
 #include <type_traits>
+#include <vector>
 
 // No implicit FP contraction: every fused multiply-add below is written out,
 // so all kernel modes (propagate, store dWaveOp, adjoint, Born) evaluate the
@@ -793,23 +795,17 @@ __device__ __forceinline__ void sample_group(const pysit_amd_batch &B, int slot,
 #ifndef PA2D_WAVES_PER_EU
 #define PA2D_WAVES_PER_EU 1
 #endif
-// With a step cursor (graph launches) the slot is cursor[0] + slot and the
-// dWaveOp store base is cursor[1]; the host checks the slot range.
+// One block of a step's grid: `b` of `nblocks` (tile blocks, then
+// `rec_blocks` receiver blocks); each of its 4 wavefronts marches one tile.
 template <typename T, int TX, int R, int MODE, bool SINGLE>
-__global__ __launch_bounds__(256, PA2D_WAVES_PER_EU) void k_step2d(const pysit_amd_batch B, const Step2DParams<T> P,
-                                                int slot, int seg_base, int n_tile_waves,
-                                                int n_rec_lanes, int rec_blocks, const int *cursor)
+__device__ __forceinline__ void step2d_block(const pysit_amd_batch &B, const Step2DParams<T> &P, int slot,
+                                             int seg_base, int n_tile_waves, int n_rec_lanes, int rec_blocks,
+                                             int b, int nblocks, T (&lds_f)[4][TX * 64],
+                                             T (&lds_z)[4][4 * zrow_len<R>()])
 {
-    __shared__ T lds_f[4][TX * 64];
-    __shared__ T lds_z[4][4 * zrow_len<R>()];
-    if (cursor) {
-        slot += cursor[0];
-        seg_base = cursor[1];
-    }
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int b = blockIdx.x;
-    const int tile_blocks = gridDim.x - rec_blocks;
+    const int tile_blocks = nblocks - rec_blocks;
 #if PA2D_REC_LAST
     const int rb = b - tile_blocks, tb = b;
 #else
@@ -831,6 +827,152 @@ __global__ __launch_bounds__(256, PA2D_WAVES_PER_EU) void k_step2d(const pysit_a
         tile2d<T, TX, R, MODE, SINGLE>(B, P, B.shots[td.x], gw, td.z & 3, td.y, td.w, slot, seg_base, lane,
                                        lds_f[w], lds_z[w]);
     }
+}
+
+// With a step cursor (graph launches) the slot is cursor[0] + slot and the
+// dWaveOp store base is cursor[1]; the host checks the slot range.
+template <typename T, int TX, int R, int MODE, bool SINGLE>
+__global__ __launch_bounds__(256, PA2D_WAVES_PER_EU) void k_step2d(const pysit_amd_batch B, const Step2DParams<T> P,
+                                                int slot, int seg_base, int n_tile_waves,
+                                                int n_rec_lanes, int rec_blocks, const int *cursor)
+{
+    __shared__ T lds_f[4][TX * 64];
+    __shared__ T lds_z[4][4 * zrow_len<R>()];
+    if (cursor) {
+        slot += cursor[0];
+        seg_base = cursor[1];
+    }
+    step2d_block<T, TX, R, MODE, SINGLE>(B, P, slot, seg_base, n_tile_waves, n_rec_lanes, rec_blocks, blockIdx.x,
+                                         gridDim.x, lds_f, lds_z);
+}
+
+// ---- the persistent time loop ----------------------------------------------
+// One cooperative launch runs every step of a call: each block takes the
+// step's blocks b = blockIdx.x, blockIdx.x + gridDim.x, ... (the grid is a
+// multiple of 8, so b lands on the XCD the per-step launch would use), then
+// all blocks meet at a grid barrier before the next step reads the new time
+// level.  The barrier is an arrival counter and a generation word in device
+// memory, with device-scope release and acquire fences around them; a block
+// that waits longer than `spin_limit` polls sets `err` and every block leaves
+// the loop at its next barrier, so the launch always ends.
+struct Persist2D {
+    int slot_begin, n_steps, direction, seg_base, window;
+    const int *counts;   // per step of the call: active tiles, active receivers
+    unsigned *bar;       // [arrived, generation], zero at launch
+    int *err;            // set on a barrier time-out
+    long long spin_limit;
+};
+
+__device__ __forceinline__ bool grid_barrier(const Persist2D &A, unsigned &gen, int &ok)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned g = gen;
+        if (atomicAdd(&A.bar[0], 1u) + 1u == gridDim.x) {
+            __hip_atomic_store(&A.bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&A.bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            long long spins = 0;
+            while (__hip_atomic_load(&A.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                if (++spins > A.spin_limit || __hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    atomicExch(A.err, 1);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        gen = g + 1;
+        __threadfence();
+        // one decision per block, read by all its threads after the barrier
+        ok = __hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+    }
+    __syncthreads();
+    return ok != 0;
+}
+
+template <typename T, int TX, int R, int MODE>
+__global__ __launch_bounds__(256, PA2D_WAVES_PER_EU) void k_persist2d(const pysit_amd_batch B,
+                                                                      const Step2DParams<T> P, const Persist2D A)
+{
+    __shared__ T lds_f[4][TX * 64];
+    __shared__ T lds_z[4][4 * zrow_len<R>()];
+    __shared__ int ok;
+    unsigned gen = 0;
+    for (int i = 0; i < A.n_steps; ++i) {
+        const int slot = A.direction > 0 ? A.slot_begin + i : A.slot_begin + A.n_steps - 1 - i;
+        const int nt = A.counts[2 * i];
+        const int nr = is_adjoint(MODE) ? 0 : A.counts[2 * i + 1];
+        const int rec_blocks = nr ? ((nr + 15) / 16 + 7) / 8 * 8 : 0;
+        const int total = (nt + 3) / 4 + rec_blocks;
+        for (int b = blockIdx.x; b < total; b += gridDim.x)
+            step2d_block<T, TX, R, MODE, false>(B, P, slot, A.seg_base, nt, nr, rec_blocks, b, total, lds_f, lds_z);
+        if (!grid_barrier(A, gen, ok)) return;
+        if constexpr (MODE == 4) {
+            if (A.window && ((i + 1) % A.window == 0 || i + 1 == A.n_steps)) {
+                // the FP32 window's flush: ic += ic32, ic32 = 0
+                const long long n = B.total_n;
+                for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+                     k += (long long)gridDim.x * blockDim.x) {
+                    B.ic[k] += (double)B.ic32[k];
+                    B.ic32[k] = 0.0f;
+                }
+                if (!grid_barrier(A, gen, ok)) return;
+            }
+        }
+    }
+}
+
+template <typename T, int TX, int R, int MODE>
+static int persist2d(const pysit_amd_batch &B, const Step2DParams<T> &P, const Persist2D &A, int max_blocks,
+                     hipStream_t st)
+{
+    auto k = k_persist2d<T, TX, R, MODE>;
+    int per_cu = 0, dev = 0;
+    PA_HIP(hipGetDevice(&dev));
+    PA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0));
+    int cus = 0;
+    PA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    // whole rounds of 8 blocks (one per XCD), so that logical block b stays
+    // on one XCD from step to step; a block beyond a step's count waits at
+    // the barrier
+    const int cap = per_cu * cus / 8 * 8;
+    PA_CHECK(cap >= 8, "the persistent loop does not fit the device");
+    const int grid = std::min(cap, (max_blocks + 7) / 8 * 8);
+    void *args[] = {(void *)&B, (void *)&P, (void *)&A};
+    PA_HIP(hipLaunchCooperativeKernel((const void *)k, dim3(grid), dim3(256), args, 0, st));
+    return PYSIT_AMD_OK;
+}
+
+template <typename T, int TX, int R>
+static int persist2d_mode(int kmode, const pysit_amd_batch &B, const Step2DParams<T> &P, const Persist2D &A,
+                          int max_blocks, hipStream_t st)
+{
+    switch (kmode) {
+    case 0: return persist2d<T, TX, R, 0>(B, P, A, max_blocks, st);
+    case 1: return persist2d<T, TX, R, 1>(B, P, A, max_blocks, st);
+    case 2: return persist2d<T, TX, R, 2>(B, P, A, max_blocks, st);
+    case 4:
+        if constexpr (std::is_same<T, float>::value) return persist2d<T, TX, R, 4>(B, P, A, max_blocks, st);
+        break;
+    default: break;
+    }
+    set_error("the persistent loop runs modes 0, 1 and 2");
+    return PYSIT_AMD_EINVAL;
+}
+
+template <typename T, int TX>
+static int persist2d_r(int r, int kmode, const pysit_amd_batch &B, const Step2DParams<T> &P, const Persist2D &A,
+                       int max_blocks, hipStream_t st)
+{
+    switch (r) {
+    case 1: return persist2d_mode<T, TX, 1>(kmode, B, P, A, max_blocks, st);
+    case 2: return persist2d_mode<T, TX, 2>(kmode, B, P, A, max_blocks, st);
+    case 3: return persist2d_mode<T, TX, 3>(kmode, B, P, A, max_blocks, st);
+    case 4: return persist2d_mode<T, TX, 4>(kmode, B, P, A, max_blocks, st);
+    }
+    set_error("order must be 2, 4, 6 or 8");
+    return PYSIT_AMD_EINVAL;
 }
 
 template <typename T, int TX, int R, int MODE>
@@ -934,15 +1076,14 @@ struct Sink2D final : StepSink {
     int flush(hipStream_t st) override { return ic_flush(B, st); }
 };
 
+// the batch steps' parameters (and the checks of the state region)
 template <typename T>
-int run2d(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int direction,
-          int seg_base, const int *active_tiles, const int *active_recs, pysit_amd_plan *plan,
-          hipStream_t st)
+static int params2d(const pysit_amd_batch &B, Step2DParams<T> &P)
 {
     PA_CHECK(B.tile_x == kTileRows2D[0] || B.tile_x == kTileRows2D[1],
              "2D tiles are " + std::to_string(kTileRows2D[0]) + " or " + std::to_string(kTileRows2D[1]) +
                  " rows along x (tile_x)");
-    Step2DParams<T> P{};
+    P = Step2DParams<T>{};
     const int r = B.order / 2;
     P.cx = axis_coef<T>(r, B.dx);
     P.cz = axis_coef<T>(r, B.dz);
@@ -973,6 +1114,18 @@ int run2d(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int 
         P.state_bytes = (int)ext;
     }
     PA_CHECK(B.total_n * 8 <= kMaxRsrcBytes, "batch too large for one launch (total_n)");
+    return PYSIT_AMD_OK;
+}
+
+template <typename T>
+int run2d(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int direction,
+          int seg_base, const int *active_tiles, const int *active_recs, pysit_amd_plan *plan,
+          hipStream_t st)
+{
+    Step2DParams<T> P;
+    const int rc = params2d<T>(B, P);
+    if (rc) return rc;
+    const int r = B.order / 2;
     const bool window = mode == 2 && B.ic32 != nullptr;
     Sink2D<T> S(B, P, r, mode, window);
     const LoopSpec L{mode, slot_begin, slot_end, direction, seg_base, active_tiles, active_recs,
@@ -984,6 +1137,81 @@ template int run2d<float>(const pysit_amd_batch &, int, int, int, int, int, cons
                           const int *, pysit_amd_plan *, hipStream_t);
 template int run2d<double>(const pysit_amd_batch &, int, int, int, int, int, const int *,
                            const int *, pysit_amd_plan *, hipStream_t);
+
+// The whole call as one persistent cooperative launch (k_persist2d).  The
+// per-step active counts go to a device buffer allocated on the stream; the
+// call waits for the launch to read the barrier's time-out flag.
+template <typename T>
+int run2d_persistent(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int direction,
+                     int seg_base, const int *active_tiles, const int *active_recs, long long spin_limit,
+                     hipStream_t st)
+{
+    Step2DParams<T> P;
+    int rc = params2d<T>(B, P);
+    if (rc) return rc;
+    PA_CHECK(mode == 0 || mode == 1 || mode == 2, "the persistent loop runs modes 0, 1 and 2");
+    PA_CHECK(!B.dw_out && !B.vol_dw, "the persistent loop has no second-order terms");
+    const int n = slot_end - slot_begin;
+    if (n <= 0) return PYSIT_AMD_OK;
+    const bool window = mode == 2 && B.ic32 != nullptr;
+    const int kmode = mode == 2 ? (window ? 4 : 2) : mode;
+    std::vector<int> host(2 * (size_t)n + 4, 0);
+    int max_blocks = 8;
+    for (int i = 0; i < n; ++i) {
+        const int j = direction > 0 ? slot_begin + i : slot_end - 1 - i;
+        const int nt = active_tiles ? active_tiles[j] : B.n_tiles;
+        const int nr = mode == 2 ? 0 : (active_recs ? active_recs[j] : B.n_rec);
+        host[2 * i] = nt;
+        host[2 * i + 1] = nr;
+        const int rec_blocks = nr ? ((nr + 15) / 16 + 7) / 8 * 8 : 0;
+        max_blocks = std::max(max_blocks, (nt + 3) / 4 + rec_blocks);
+    }
+    void *buf = nullptr;
+    PA_HIP(hipMallocAsync(&buf, host.size() * sizeof(int), st));
+    int *dcounts = (int *)buf;
+    unsigned *bar = (unsigned *)(dcounts + 2 * n);
+    int *err = (int *)(bar + 2);
+    rc = PYSIT_AMD_OK;
+    if (hipMemcpyAsync(buf, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess) {
+        set_error("copy of the persistent loop's step counts failed");
+        rc = PYSIT_AMD_EDEVICE;
+    }
+    if (!rc) {
+        Persist2D A{};
+        A.slot_begin = slot_begin;
+        A.n_steps = n;
+        A.direction = direction;
+        A.seg_base = seg_base;
+        A.window = window ? B.ic_window : 0;
+        A.counts = dcounts;
+        A.bar = bar;
+        A.err = err;
+        A.spin_limit = spin_limit;
+        rc = B.tile_x == kTileRows2D[0] ? persist2d_r<T, kTileRows2D[0]>(B.order / 2, kmode, B, P, A, max_blocks, st)
+                                        : persist2d_r<T, kTileRows2D[1]>(B.order / 2, kmode, B, P, A, max_blocks, st);
+    }
+    int flag = 0;
+    if (!rc && hipMemcpyAsync(&flag, err, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) {
+        set_error("read of the persistent loop's barrier flag failed");
+        rc = PYSIT_AMD_EDEVICE;
+    }
+    const hipError_t se = hipStreamSynchronize(st);
+    (void)hipFreeAsync(buf, st);
+    if (!rc && se != hipSuccess) {
+        set_error(std::string("persistent loop: ") + hipGetErrorString(se));
+        rc = PYSIT_AMD_EDEVICE;
+    }
+    if (!rc && flag) {
+        set_error("persistent loop: a grid barrier timed out");
+        rc = PYSIT_AMD_EDEVICE;
+    }
+    return rc;
+}
+
+template int run2d_persistent<float>(const pysit_amd_batch &, int, int, int, int, int, const int *, const int *,
+                                     long long, hipStream_t);
+template int run2d_persistent<double>(const pysit_amd_batch &, int, int, int, int, int, const int *, const int *,
+                                      long long, hipStream_t);
 
 // One step with the reference contract (dense rhs, caller-owned outputs).
 template <typename T>

@@ -305,6 +305,12 @@ class DeviceBatch:
         if graph_steps is not None and int(graph_steps) < 0:
             raise ValueError("graph_steps must be >= 0")
         self.plan = None
+        # 2D time loops without second-order terms as one persistent
+        # cooperative launch per run (pysit_amd_batch_run_persistent) in
+        # place of step launches or graphs.  The results are the same; its
+        # device-wide barrier between steps costs more than a graph's launch
+        # boundary on MI355X, hence off by default.
+        self.persistent = False
 
         if isinstance(model, torch.Tensor):
             self.model = model.detach().to(device=self.dev, dtype=self.tdtype).contiguous()
@@ -796,7 +802,10 @@ class DeviceBatch:
             ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
             ev[0].record()
         atp, arp = at.ctypes.data_as(ctypes.c_void_p), ar.ctypes.data_as(ctypes.c_void_p)
-        if self.graph_steps:
+        if self.persistent and self.dim == 2 and mode in (0, 1, 2) and not (b.dw_out or b.vol_dw):
+            N.call("pysit_amd_batch_run_persistent", ctypes.byref(b), mode, j0, j1, direction, seg_base, atp, arp,
+                   _stream())
+        elif self.graph_steps:
             if self.plan is None:
                 self.plan = N.Plan(self.graph_steps)
             self.plan.run(b, mode, j0, j1, direction, seg_base, atp, arp, _stream())
