@@ -12,6 +12,10 @@ condition and the PML fold, through the torch.autograd.Function wrapper.
 
 value  = 2 * sum_shots(N_pad * nt) / seconds per gradient  (Gpt-updates/s),
          recompute steps of a checkpoint schedule are not counted.
+checkpointed = the same batched gradient under an 8 GiB memory budget: the
+         forward keeps checkpoints, the adjoint recomputes each segment's
+         dWaveOp; its gradient equals the store-all one to FP32 rounding
+         (the FP32 imaging windows restart at each segment).
 plain  = the same gradient with one shot per launch (16 shot loops).
 roofline: the adjoint kernel (the dominant one): algorithmic bytes per
          launch (FP32 fields: u_k, u_k-1, C read, u_k+1 written = 16 B per
@@ -51,6 +55,8 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=6.0, help="target seconds per CPU baseline leg")
     p.add_argument("--tile-x", type=int, default=None)
     p.add_argument("--memory-budget-gb", type=float, default=None)
+    p.add_argument("--ckpt-budget-gb", type=float, default=8.0,
+                   help="c2: memory budget of the checkpointed gradient leg (0: skip it)")
     p.add_argument("--profile-run", action="store_true", help="minimal run for rocprofv3 (no plain, no CPU)")
     p.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5"], default="c2",
                    help="c2: the headline 16-shot gradient; c3: 64 shots of uneven aperture and length; "
@@ -763,6 +769,38 @@ def main():
         "misfit": float(loss.detach()), "grad_norm": gnorm,
     }
 
+    if args.ckpt_budget_gb and not (args.profile_run or args.pmc_run):
+        # the same gradient with the dWaveOp history checkpointed: the budget
+        # holds a store of K slots and the checkpoints, and the adjoint
+        # recomputes each segment's dWaveOp (those launches are not counted
+        # in the rate).  The FP32 imaging windows restart at every segment,
+        # so the gradient equals the store-all one to FP32 rounding (bit for
+        # bit with FP64 imaging)
+        g_all = m.grad.detach().clone()
+        problem.batch = None
+        torch.cuda.empty_cache()
+        ck_budget = int(args.ckpt_budget_gb * 2 ** 30)
+        ck = ShotProblem(solver, shots, memory_budget=ck_budget, tile_x=args.tile_x)
+        for _ in range(args.warmup):
+            gradient(ck)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            gradient(ck)
+        torch.cuda.synchronize()
+        t_ck = (time.perf_counter() - t0) / args.steps
+        b = ck.batch
+        result["checkpointed"] = {
+            "value": updates / t_ck / 1e9, "unit": "Gpt-updates/s", "ms_per_step": t_ck * 1e3,
+            "memory_budget_gib": args.ckpt_budget_gb, "dwaveop_segment": b.K, "levels": [int(v) for v in b.levels],
+            "schedule": "binomial" if b.snaps is not None else "nested segments",
+            "gradient_vs_store_all_rel_l2": float(torch.linalg.norm((m.grad - g_all).double())
+                                                  / torch.linalg.norm(g_all.double())),
+            "gradient_roofline_frac": grad_bytes / t_ck / 1e9 / HBM_PEAK_GBS,
+            "note": "rate counts 2 N nt per shot; the dWaveOp recompute launches are extra work"}
+        ck.batch = None
+        del ck, b
+        torch.cuda.empty_cache()
     if args.pmc_run:
         b = problem.batch
         print(json.dumps({"pmc_run": "done", "nt": nt, "shots": len(shots), "tile_x": b.tile_x,
@@ -805,6 +843,8 @@ def main():
         result["plain"] = {"value": updates / t_plain / 1e9, "ms_per_gradient": t_plain * 1e3,
                            "form": "one shot per launch, same kernels, 16 sequential shot gradients"}
         result["batched_vs_plain"] = t_plain / t_step
+        if "checkpointed" in result:
+            result["checkpointed"]["vs_plain"] = t_plain / (result["checkpointed"]["ms_per_step"] * 1e-3)
 
     if not (args.no_cpu or args.profile_run):
         kp = solver.kernel_pml()
