@@ -595,7 +595,9 @@ def run_fwi(args):
     def band_summary(reports):
         return [{"f0_hz": r["f0_hz"], "objective_first": r["objective"][0], "objective_final": r["objective"][-1],
                  "misfit_reduction": r["misfit_reduction"], "model_error_start": r["model_error_start"],
-                 "model_error_final": r["model_error_final"], "evaluations": r["evaluations"],
+                 "model_error_final": r["model_error_final"],
+                 "model_error_upper_third_start": r["model_error_shallow_start"],
+                 "model_error_upper_third_final": r["model_error_shallow_final"], "evaluations": r["evaluations"],
                  "gradients": r["gradients"], "seconds": r["seconds"]} for r in reports]
     line = {
         "metric": "C5 multi-band FWI loop Gpt-updates/s (forward, gradient and line-search evaluations, "
@@ -610,6 +612,8 @@ def run_fwi(args):
         "fp32": {"seconds": s32["seconds"], "updates_gpt": updates / 1e9,
                  "model_error_start": s32["start_error"],
                  "model_error_final": model_error(s32["model"], s32["true"]),
+                 "model_error_upper_third_start": s32["reports"][0]["model_error_shallow_start"],
+                 "model_error_upper_third_final": s32["reports"][-1]["model_error_shallow_final"],
                  "bands": band_summary(s32["reports"])},
     }
     if "double" in out:

@@ -221,9 +221,9 @@ class _CountedObjective(object):
     algorithmic grid-point updates (forward: shots x padded nodes x steps at
     the model's dt; gradient: twice that)."""
 
-    def __init__(self, solver, n_nodes, **kwargs):
+    def __init__(self, solver, n_nodes, inner=None, **kwargs):
         from .objective_functions import TemporalLeastSquares
-        self.inner = TemporalLeastSquares(solver, **kwargs)
+        self.inner = inner if inner is not None else TemporalLeastSquares(solver, **kwargs)
         self.solver = solver
         self.n_nodes = n_nodes
         self.evaluations = self.gradients = 0
@@ -249,14 +249,23 @@ class _CountedObjective(object):
         return g
 
 
-def model_error(model, true_model):
-    """Relative L2 error of the wave speed over the unpadded domain."""
-    c, ct = np.asarray(model.C).ravel(), np.asarray(true_model.C).ravel()
+def model_error(model, true_model, depth_fraction=None):
+    """Relative L2 error of the wave speed over the unpadded domain, or over
+    its upper ``depth_fraction`` of depth (the last axis)."""
+    c, ct = np.asarray(model.C), np.asarray(true_model.C)
+    if depth_fraction is not None:
+        shape = model.mesh.shape(as_grid=True)
+        nz = shape[-1]
+        keep = max(1, int(round(depth_fraction * nz)))
+        c = c.reshape(shape)[..., :keep]
+        ct = ct.reshape(shape)[..., :keep]
+    c, ct = c.ravel(), ct.ravel()
     return float(np.linalg.norm(c - ct) / np.linalg.norm(ct))
 
 
 def multiband_fwi(mesh, solver, true_model, start_model, bands=(3.0, 5.0, 8.0), iterations=20,
-                  memory_length=10, n_shots=32, depth=None, verbose=False, sync=None, log=None):
+                  memory_length=10, n_shots=32, depth=None, verbose=False, sync=None, log=None,
+                  objective=None, generate=None, shallow=1.0 / 3.0):
     """Frequency-continuation FWI with LBFGS over TemporalLeastSquares.
 
     For each Ricker peak frequency in ``bands`` (low to high): new shots with
@@ -274,9 +283,13 @@ def multiband_fwi(mesh, solver, true_model, start_model, bands=(3.0, 5.0, 8.0), 
 
     ``sync`` (optional) is called before each band's timer is read, e.g. a
     device synchronisation; ``log`` (a file) takes the LBFGS progress lines
-    when ``verbose``.  Returns (final model, one report dict per band:
-    frequency, objective history, misfit reduction, model error before and
-    after, evaluations, gradients, grid-point updates, seconds)."""
+    when ``verbose``.  ``objective(solver)`` and ``generate(shots, solver,
+    model)`` replace TemporalLeastSquares and generate_seismic_data (another
+    implementation of the same objective, run through the same loop).
+    Returns (final model, one report dict per band: frequency, objective
+    history, misfit reduction, model error before and after over the whole
+    model and over its upper ``shallow`` fraction of depth, evaluations,
+    gradients, grid-point updates, seconds)."""
     import time
     from .modeling import generate_seismic_data
     from .optimization import LBFGS
@@ -288,10 +301,11 @@ def multiband_fwi(mesh, solver, true_model, start_model, bands=(3.0, 5.0, 8.0), 
         t0 = time.perf_counter()
         shots = equispaced_acquisition(mesh, RickerWavelet(f0), sources=n_shots, source_depth=depth,
                                        receivers="max", receiver_depth=depth)
-        generate_seismic_data(shots, solver, true_model)
-        obj = _CountedObjective(solver, n_nodes)
+        (generate or generate_seismic_data)(shots, solver, true_model)
+        obj = _CountedObjective(solver, n_nodes, inner=objective(solver) if objective is not None else None)
         obj.updates += len(shots) * n_nodes * solver.nsteps
         err0 = model_error(model, true_model)
+        err0_shallow = model_error(model, true_model, shallow)
         invalg = LBFGS(obj, memory_length=memory_length)
         if log is not None:
             invalg.logfile = log
@@ -307,6 +321,8 @@ def multiband_fwi(mesh, solver, true_model, start_model, bands=(3.0, 5.0, 8.0), 
             "f0_hz": float(f0), "iterations": int(iterations), "objective": hist + [float(f_final)],
             "misfit_reduction": float(f_final / hist[0]) if hist and hist[0] > 0 else None,
             "model_error_start": err0, "model_error_final": model_error(model, true_model),
+            "shallow_fraction": float(shallow), "model_error_shallow_start": err0_shallow,
+            "model_error_shallow_final": model_error(model, true_model, shallow),
             "evaluations": obj.evaluations, "gradients": obj.gradients, "updates": obj.updates,
             "seconds": time.perf_counter() - t0})
     return model, reports

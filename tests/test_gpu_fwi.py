@@ -120,6 +120,81 @@ def test_multiband_fwi_fp32_follows_fp64(gpu, fwi_runs):
     assert err < 1e-3
 
 
+class _OracleLeastSquares(object):
+    """TemporalLeastSquares restated on the oracle (the FP64 C loops, on the
+    host cores): 1/2 dt sum ||interp(d_obs) - S u||^2 per shot, and its
+    gradient -sum_shots migrate(residual)
+    (objective_functions/temporal_least_squares.py:56-84, :141-198)."""
+
+    class _Serial(object):
+        use_parallel = False
+        rank = 0
+
+    parallel_wrap_shot = _Serial()
+
+    def __init__(self, solver):
+        self.solver = solver
+
+    def use_parallel(self):
+        return False
+
+    def _residuals(self, shots, m0):
+        from tests.problems import oracle_from_solver
+        self.solver.model_parameters = m0
+        ts = self.solver.ts()
+        for sh in shots:
+            op = oracle_from_solver(self.solver, sh)
+            yield op, sh.receivers.interpolate_data(ts) - op.forward()["simdata"]
+
+    def evaluate(self, shots, m0, **kwargs):
+        r2 = sum(np.linalg.norm(r) ** 2 for _, r in self._residuals(shots, m0))
+        return 0.5 * r2 * self.solver.dt
+
+    def compute_gradient(self, shots, m0, aux_info={}, **kwargs):
+        g, r2 = 0.0, 0.0
+        for op, r in self._residuals(shots, m0):
+            g = g - op.migrate(r)
+            r2 += np.linalg.norm(r) ** 2
+        if "objective_value" in aux_info and aux_info["objective_value"][0]:
+            aux_info["objective_value"] = (True, 0.5 * r2 * self.solver.dt)
+        return m0.perturbation(data=np.asarray(g).reshape(-1, 1))
+
+
+def _oracle_data(shots, solver, model):
+    """generate_seismic_data through the oracle: each shot's traces at the
+    model's own time axis (modeling/data_modeling.py:10-60)."""
+    from tests.problems import oracle_from_solver
+    solver.model_parameters = model
+    ts = solver.ts()
+    for sh in shots:
+        d = oracle_from_solver(solver, sh).forward()["simdata"]
+        sh.reset_time_series(ts)
+        sh.dt = solver.dt
+        sh.trange = solver.trange
+        sh.receivers.data = np.asarray(d, dtype=np.float64)
+
+
+def test_multiband_fwi_fp64_follows_the_oracle_on_the_host(gpu, fwi_runs):
+    """The same two-band LBFGS loop with the oracle's FP64 host loops in
+    place of the device (data, objective and gradient): every objective
+    value of the loop and the final model agree with the FP64 device run's,
+    so the device loop is the host loop with its propagations moved."""
+    from pysit_amd.workloads import multiband_fwi
+    mesh, solver, true_model, start_model = _small_fwi("double")
+    m_cpu, r_cpu = multiband_fwi(mesh, solver, true_model, start_model, bands=(5.0, 8.0), iterations=3,
+                                 n_shots=6, depth=30.0, objective=_OracleLeastSquares, generate=_oracle_data)
+    m_gpu, r_gpu = fwi_runs["double"]
+    for a, b in zip(r_gpu, r_cpu):
+        print(a["objective"], b["objective"], flush=True)
+        assert len(a["objective"]) == len(b["objective"])
+        assert np.allclose(a["objective"], b["objective"], rtol=1e-8, atol=0)
+        assert a["evaluations"] == b["evaluations"] and a["gradients"] == b["gradients"]
+    c_gpu, c_cpu = np.asarray(m_gpu.C).ravel(), np.asarray(m_cpu.C).ravel()
+    err = np.linalg.norm(c_gpu - c_cpu) / np.linalg.norm(c_cpu)
+    print("final model device vs host FP64", err, flush=True)
+    assert err < 1e-9
+
+
 def test_observed_on_device_matches_host_interpolation(gpu):
     """ObservedOnDevice restates interp1d(kind='linear', fill_value=0): the
     same values as the host path, on a finer and a longer time axis (zero
