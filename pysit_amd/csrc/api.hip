@@ -23,7 +23,7 @@ int run2d_persistent(const pysit_amd_batch &, int, int, int, int, int, const int
 
 int tile_rows_2d(int dtype);
 bool tile_rows_2d_supported(int tx);
-int tile_rows_3d(int axis);
+int tile_rows_3d(int dtype, int axis);
 
 template <typename T>
 int step2d_entry(int, const T *, const T *, const T *, const T *, const T *, const T *, const T *,
@@ -200,7 +200,7 @@ int pysit_amd_tile_rows(int dim, int dtype, int axis)
         return PYSIT_AMD_EINVAL;
     }
     if (dim == 2) return axis == 0 ? pa::tile_rows_2d(dtype) : 1;
-    return pa::tile_rows_3d(axis);
+    return pa::tile_rows_3d(dtype, axis);
 }
 
 int pysit_amd_tile_rows_supported(int dim, int dtype, int tile_x)
@@ -211,7 +211,7 @@ int pysit_amd_tile_rows_supported(int dim, int dtype, int tile_x)
         return PYSIT_AMD_EINVAL;
     }
     if (dim == 2) return pa::tile_rows_2d_supported(tile_x) ? 1 : 0;
-    return tile_x == pa::tile_rows_3d(0) ? 1 : 0;
+    return tile_x == pa::tile_rows_3d(dtype, 0) ? 1 : 0;
 }
 
 int pysit_amd_abi_check(int shot_size, int batch_size)

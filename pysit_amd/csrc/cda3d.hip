@@ -46,7 +46,13 @@ constexpr bool ic64_3(int mode) { return mode == 2 || mode == 6; }
 template <int MODE> struct Acc3D { typedef double type; };
 template <> struct Acc3D<4> { typedef float type; };
 
-constexpr int TY3 = 8;  // y rows per workgroup (one wavefront each)
+// y rows per workgroup (one wavefront each): the waves 0..2r-1 also load the
+// 2r y-halo rows.  16 rows in FP32 (1024-thread workgroups, one per CU, half
+// the y-halo reads per node) ran both 3D steps 11 % slower than 8.
+#ifndef PA3D_TY_F32
+#define PA3D_TY_F32 8
+#endif
+template <typename T> struct Tile3 { static constexpr int TY = sizeof(T) == 4 ? PA3D_TY_F32 : 8; };
 // x planes per tile: 16 runs the 3D steps 7-14 % faster than 8 (the x halo
 // of 2r planes and each tile's start are spread over twice the planes); the
 // injection block in LDS grows to TX3 * 8 * 64 values (65 KB per FP32
@@ -56,7 +62,7 @@ constexpr int TY3 = 8;  // y rows per workgroup (one wavefront each)
 #endif
 constexpr int TX3 = PA3D_TX;
 
-int tile_rows_3d(int axis) { return axis == 0 ? TX3 : TY3; }
+int tile_rows_3d(int dtype, int axis) { return axis == 0 ? TX3 : (dtype ? Tile3<double>::TY : Tile3<float>::TY); }
 
 // x planes whose loads are in flight ahead of the plane being computed: 1
 // in PML tiles (their march is at the 128-VGPR limit; 2 spills), 2 in plain
@@ -91,12 +97,13 @@ struct Step3DParams {
 template <typename T, int R>
 struct Lds3D {
     static constexpr int LZ = 64 + 2 * R;
-    static constexpr int LY = TY3 + 2 * R;
+    static constexpr int TY = Tile3<T>::TY;
+    static constexpr int LY = TY + 2 * R;
     T u[2][LY * LZ];
     T ps[2][LY * LZ];
     T py[2][LY * 64];
-    T pz[2][TY3 * LZ];
-    T f[TX3 * TY3 * 64];
+    T pz[2][TY * LZ];
+    T f[TX3 * TY * 64];
     T junk[4][64];
 };
 
@@ -526,6 +533,7 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
     const int zs = S.zs;
     const int z0 = (tz == ntz - 1) ? zlast : tz * 64;
     const int own_hi = (tz == ntz - 1) ? nz : min(z0 + 64, zlast);
+    constexpr int TY3 = Tile3<T>::TY;
     const int y0 = tyi * TY3, x0 = tx * TX;
     const int z = z0 + lane, y = y0 + w;
     c.y = y;
@@ -694,7 +702,8 @@ __device__ __forceinline__ void sample_group3(const pysit_amd_batch &B, int slot
 // INC: the batch's increment form (du); false: the reference-contract step
 // (u_{k-1} in the second u buffer)
 template <typename T, int R, int MODE, bool INC>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? PA3D_WAVES_PER_EU : 2)))
+__global__ __launch_bounds__(Tile3<T>::TY * 64)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? PA3D_WAVES_PER_EU : 2)))
 void k_step3d(const pysit_amd_batch B, const Step3DParams<T> P,
                                                 int slot, int seg_base, int n_tile_blocks,
                                                 int n_rec_lanes, int rec_blocks, const int *cursor)
@@ -706,6 +715,7 @@ void k_step3d(const pysit_amd_batch B, const Step3DParams<T> P,
         seg_base = cursor[1];
     }
     // the tile blocks, then the receiver blocks (they fill the launch's tail)
+    constexpr int TY3 = Tile3<T>::TY;
     const int b = blockIdx.x;
     const int tile_blocks = gridDim.x - rec_blocks;
     if (!adj3(MODE) && b >= tile_blocks) {
@@ -729,6 +739,7 @@ template <typename T, int R, int MODE>
 static int launch3d(const pysit_amd_batch &B, const Step3DParams<T> &P, int slot, int seg_base,
                     int n_tiles, int n_recs, hipStream_t st, const int *cursor = nullptr)
 {
+    constexpr int TY3 = Tile3<T>::TY;
     const int recs = adj3(MODE) ? 0 : n_recs;
     const int rec_blocks = recs ? (recs + TY3 * 4 - 1) / (TY3 * 4) : 0;
     const int blocks = n_tiles + rec_blocks;
@@ -799,6 +810,7 @@ int run3d(const pysit_amd_batch &B, int mode, int slot_begin, int slot_end, int 
           int seg_base, const int *active_tiles, const int *active_recs, pysit_amd_plan *plan,
           hipStream_t st)
 {
+    constexpr int TY3 = Tile3<T>::TY;
     PA_CHECK(B.tile_x == TX3 && B.tile_y == TY3,
              "3D tiles are " + std::to_string(TX3) + " x " + std::to_string(TY3) + " (tile_x, tile_y)");
     // every per-batch array (the FP64 accumulator at 8 B per node the
@@ -835,6 +847,7 @@ int step3d_entry(int order, const T *km1_u, const T *k_phix, const T *k_phiy, co
                  int ny, int nz, T *kp1_phix, T *kp1_phiy, T *kp1_phiz, T *kp1_psi, T *kp1_u,
                  hipStream_t st)
 {
+    constexpr int TY3 = Tile3<T>::TY;
     PA_CHECK(order == 2 || order == 4 || order == 6 || order == 8, "order must be 2, 4, 6 or 8");
     PA_CHECK(nx >= 1 && ny >= 1 && nz >= 1, "grid must be non-empty");
     PA_CHECK(km1_u && k_phix && k_phiy && k_phiz && k_psi && k_u && C && kp1_phix && kp1_phiy &&
