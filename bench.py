@@ -14,8 +14,7 @@ value  = 2 * sum_shots(N_pad * nt) / seconds per gradient  (Gpt-updates/s),
          recompute steps of a checkpoint schedule are not counted.
 checkpointed = the same batched gradient under an 8 GiB memory budget: the
          forward keeps checkpoints, the adjoint recomputes each segment's
-         dWaveOp; its gradient equals the store-all one to FP32 rounding
-         (the FP32 imaging windows restart at each segment).
+         dWaveOp; its gradient equals the store-all one bit for bit.
 plain  = the same gradient with one shot per launch (16 shot loops).
 roofline: the adjoint kernel (the dominant one): algorithmic bytes per
          launch (FP32 fields: u_k, u_k-1, C read, u_k+1 written = 16 B per
@@ -777,9 +776,8 @@ def main():
         # the same gradient with the dWaveOp history checkpointed: the budget
         # holds a store of K slots and the checkpoints, and the adjoint
         # recomputes each segment's dWaveOp (those launches are not counted
-        # in the rate).  The FP32 imaging windows restart at every segment,
-        # so the gradient equals the store-all one to FP32 rounding (bit for
-        # bit with FP64 imaging)
+        # in the rate).  The FP32 imaging windows follow the slots, so the
+        # gradient equals the store-all one bit for bit
         g_all = m.grad.detach().clone()
         problem.batch = None
         torch.cuda.empty_cache()

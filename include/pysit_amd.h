@@ -201,9 +201,13 @@ typedef struct {
     const void *born_m1;
     /* two-level imaging accumulator (FP32 batches, adjoint): when ic32 is
      * not NULL the per-step products are summed into ic32 (FP32, total_n,
-     * zero on entry) and ic32 is added into ic (FP64) and cleared every
-     * ic_window slots and at the end of every pysit_amd_batch_run call, so ic
-     * holds the complete sum after each call.  NULL: every step adds into ic */
+     * zero at the start of a sweep) and ic32 is added into ic (FP64) and
+     * cleared after every slot j with j % ic_window == 0 (backward runs).
+     * The windows follow the slots, not the calls: a sweep split into calls
+     * (checkpoint segments) sums exactly as one call, and a call that stops
+     * inside a window leaves it open in ic32 for the next call.  A backward
+     * sweep ends on slot 0, so ic holds the complete sum after it.
+     * NULL: every step adds into ic                                         */
     float *ic32;
     int ic_window;
     /* precomputed injection (optional, any mode): when inj_pre is not NULL,
@@ -297,7 +301,9 @@ int pysit_amd_batch_run_persistent(const pysit_amd_batch *b, int mode, int slot_
  *     (total_n); larger batches are refused with PYSIT_AMD_EINVAL;
  *   - every call on one GPU shares that GPU's array: calls that overlap on
  *     one GPU (two streams, two threads) race.  Each call clears the part it
- *     uses before its first step (with a kernel, not a memset).           */
+ *     uses before its first step (with a kernel, not a memset), and a call
+ *     that stops inside an FP32 window adds that window into ic at its end
+ *     (pysit_amd_batch_run leaves it open in ic32 for the next call).     */
 int pysit_amd_batch_run_static(const pysit_amd_batch *b, int mode, int slot_begin, int slot_end,
                                int direction, int seg_base, const int *active_tiles,
                                const int *active_recs, void *stream);

@@ -21,6 +21,7 @@ template <typename T>
 int run2d_persistent(const pysit_amd_batch &, int, int, int, int, int, const int *, const int *, long long,
                      hipStream_t);
 
+int ic_flush(const pysit_amd_batch &B, hipStream_t st);
 int tile_rows_2d(int dtype);
 bool tile_rows_2d_supported(int tx);
 int tile_rows_3d(int dtype, int axis);
@@ -459,8 +460,14 @@ int pysit_amd_batch_run_static(const pysit_amd_batch *b, int mode, int slot_begi
         }
         B.ic32 = p;
     }
-    return batch_run(&B, mode, slot_begin, slot_end, direction, seg_base, active_tiles, active_recs, nullptr,
-                     stream);
+    int rc = batch_run(&B, mode, slot_begin, slot_end, direction, seg_base, active_tiles, active_recs, nullptr,
+                       stream);
+    // the next call clears the array: a run that stops inside a window adds
+    // its open window into ic now
+    const int last = direction > 0 ? slot_end - 1 : slot_begin;
+    if (!rc && window && slot_end > slot_begin && !window_end(last, direction, B.ic_window))
+        rc = ic_flush(B, (hipStream_t)stream);
+    return rc;
 }
 
 int pysit_amd_plan_run(pysit_amd_plan *plan, const pysit_amd_batch *b, int mode, int slot_begin,

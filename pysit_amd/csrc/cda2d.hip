@@ -909,7 +909,7 @@ __global__ __launch_bounds__(256, PA2D_WAVES_PER_EU) void k_persist2d(const pysi
             step2d_block<T, TX, R, MODE, false>(B, P, slot, A.seg_base, nt, nr, rec_blocks, b, total, lds_f, lds_z);
         if (!grid_barrier(A, gen, ok)) return;
         if constexpr (MODE == 4) {
-            if (A.window && ((i + 1) % A.window == 0 || i + 1 == A.n_steps)) {
+            if (A.window && window_end(slot, A.direction, A.window)) {
                 // the FP32 window's flush: ic += ic32, ic32 = 0
                 const long long n = B.total_n;
                 for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;

@@ -277,8 +277,18 @@ struct StepSink {
 struct LoopSpec {
     int mode, slot_begin, slot_end, direction, seg_base;
     const int *active_tiles, *active_recs;
-    int window;  // flush every `window` steps of the call and after its last (0: never)
+    int window;  // FP32 imaging window: flush after each slot that ends one (0: never)
 };
+
+// The FP32 imaging windows follow the batch's slots, not the calls: a window
+// ends after slot j when j % w == 0 in a backward run ((j + 1) % w == 0 in a
+// forward one), so a sweep split into calls (checkpoint segments, binomial
+// replays) groups its sums exactly as one call does.  A backward sweep ends
+// on slot 0, which ends a window.
+__host__ __device__ inline bool window_end(int j, int direction, int w)
+{
+    return direction < 0 ? j % w == 0 : (j + 1) % w == 0;
+}
 
 // Runs the steps of `L` on `st`: one launch per step, or, with a plan, in
 // graph chunks of the plan's step count (see plan.hip).

@@ -52,18 +52,27 @@ static void free_entry(pysit_amd_plan::Entry &e)
     e.graph = nullptr;
 }
 
-static std::vector<unsigned char> graph_key(const pysit_amd_batch &B, const LoopSpec &L, int nt, int nr, int G)
+static std::vector<unsigned char> graph_key(const pysit_amd_batch &B, const LoopSpec &L, int nt, int nr, int G,
+                                            int phase)
 {
-    const int tail[6] = {L.mode, L.direction, nt, nr, L.window, G};
+    const int tail[7] = {L.mode, L.direction, nt, nr, L.window, G, phase};
     std::vector<unsigned char> k(sizeof(B) + sizeof(tail));
     std::memcpy(k.data(), &B, sizeof(B));
     std::memcpy(k.data() + sizeof(B), tail, sizeof(tail));
     return k;
 }
 
+// The slot of local step i of a chunk, modulo the window, whose first slot
+// is `phase` modulo the window; whether a window ends after it.
+static bool chunk_window_end(int phase, int i, int direction, int w)
+{
+    const int j = ((phase + direction * i) % w + w) % w;
+    return window_end(j, direction, w);
+}
+
 // The cached graph for `key`, captured on first use.
 static int graph_for(pysit_amd_plan *plan, const std::vector<unsigned char> &key, const LoopSpec &L, StepSink &S,
-                     int nt, int nr, hipStream_t st, hipGraphExec_t *out)
+                     int nt, int nr, int phase, hipStream_t st, hipGraphExec_t *out)
 {
     for (auto &e : plan->graphs) {
         if (e.key == key) {
@@ -86,7 +95,7 @@ static int graph_for(pysit_amd_plan *plan, const std::vector<unsigned char> &key
     int rc = PYSIT_AMD_OK;
     for (int i = 0; i < G && !rc; ++i) {
         rc = S.step(L.direction * i, 0, nt, nr, plan->cursor, plan->cap);
-        if (!rc && L.window && (i + 1) % L.window == 0) rc = S.flush(plan->cap);
+        if (!rc && L.window && chunk_window_end(phase, i, L.direction, L.window)) rc = S.flush(plan->cap);
     }
     if (!rc) {
         hipLaunchKernelGGL(k_cursor_add, dim3(1), dim3(64), 0, plan->cap, plan->cursor, L.direction * G);
@@ -147,9 +156,12 @@ int time_loop(pysit_amd_plan *plan, const pysit_amd_batch &B, const LoopSpec &L,
         const int j0 = slot_of(i0);
         const int nt = tiles_at(j0), nr = recs_at(j0);
         bool as_graph = uniform(i0, nt, nr);
+        // window phase of the chunk's first slot: graphs of chunks at another
+        // phase flush after other local steps
+        const int phase = L.window ? j0 % L.window : 0;
         std::vector<unsigned char> key;
         if (as_graph) {
-            key = graph_key(B, L, nt, nr, G);
+            key = graph_key(B, L, nt, nr, G, phase);
             bool cached = false;
             for (auto &e : plan->graphs) cached = cached || e.key == key;
             // a new graph pays for its capture only if it is replayed
@@ -160,7 +172,7 @@ int time_loop(pysit_amd_plan *plan, const pysit_amd_batch &B, const LoopSpec &L,
             for (int i = i0; i < i0 + len; ++i) {
                 const int j = slot_of(i);
                 int rc = S.step(j, L.seg_base, tiles_at(j), recs_at(j), nullptr, st);
-                if (!rc && L.window && ((i + 1) % L.window == 0 || i + 1 == n)) rc = S.flush(st);
+                if (!rc && L.window && window_end(j, L.direction, L.window)) rc = S.flush(st);
                 if (rc) return rc;
             }
             if (plan) plan->n_direct += len;
@@ -168,7 +180,7 @@ int time_loop(pysit_amd_plan *plan, const pysit_amd_batch &B, const LoopSpec &L,
             continue;
         }
         hipGraphExec_t x = nullptr;
-        int rc = graph_for(plan, key, L, S, nt, nr, st, &x);
+        int rc = graph_for(plan, key, L, S, nt, nr, phase, st, &x);
         if (rc) return rc;
         if (!cursor_known || cursor_slot != j0) {
             hipLaunchKernelGGL(k_cursor_set, dim3(1), dim3(64), 0, st, plan->cursor, j0, L.seg_base);
