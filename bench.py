@@ -359,6 +359,9 @@ def run_c1(args):
                                f"T 1.5 (nt {nt} at the start model's dt), FP64",
                    "nt": nt, "grid_padded": list(shape)},
         "vs_oracle": {"traces_rel_l2": rel(sim, ref["simdata"]), "gradient_rel_l2": rel(g.data, grad_ref)},
+        "residency": f"the state ({7 * n * 8 / 2**20:.1f} MiB of FP64 fields) and the dWaveOp history "
+                     f"({nt * n * 8 / 2**20:.0f} MiB) sit in the 256 MB Infinity Cache; each step is one launch "
+                     f"over {n} nodes, bound by latency, not by HBM",
     }
     if not args.no_cpu:
         cpu = cpu_gradient(solver, shot, 20.0)
@@ -769,6 +772,9 @@ def main():
         "traffic_source": traffic.get("source"),
         "gradient_roofline_frac": grad_bytes / t_step / 1e9 / HBM_PEAK_GBS,
         "bytes_per_update": grad_bytes / updates,
+        "residency": (f"HBM: the {len(shots)} shots' state ({problem.batch.state_bytes() / 2**20:.0f} MiB) and "
+                      f"the dWaveOp store ({segment * problem.batch.total_n * 4 / 2**30:.1f} GiB) exceed the "
+                      f"256 MB Infinity Cache"),
         "misfit": float(loss.detach()), "grad_norm": gnorm,
     }
 
