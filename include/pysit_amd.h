@@ -281,8 +281,9 @@ int pysit_amd_batch_run(const pysit_amd_batch *b, int mode, int slot_begin, int 
  * replaces, so the graph loop of pysit_amd_plan_run stays the faster
  * choice for small batches.  The call allocates a
  * small device buffer on the stream (the per-step counts and the barrier)
- * and waits for the launch to end; a barrier that waits for seconds ends the
- * launch with PYSIT_AMD_EDEVICE. */
+ * and waits for the launch to end; a barrier that waits for seconds (2^24
+ * polls, or the environment's PYSIT_AMD_PERSIST_SPIN_LIMIT) ends the launch
+ * with PYSIT_AMD_EDEVICE. */
 int pysit_amd_batch_run_persistent(const pysit_amd_batch *b, int mode, int slot_begin, int slot_end,
                                    int direction, int seg_base, const int *active_tiles,
                                    const int *active_recs, void *stream);

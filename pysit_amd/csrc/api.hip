@@ -3,6 +3,7 @@
 // error strings, and the PML fold kernel.
 #include "common.h"
 
+#include <cstdlib>
 #include <mutex>
 
 namespace pa {
@@ -427,9 +428,17 @@ int pysit_amd_batch_run_persistent(const pysit_amd_batch *b, int mode, int slot_
 {
     clear_error();
     // a barrier wait longer than 2^24 polls of about 0.1 us (seconds, where a
-    // step takes microseconds) ends the launch with an error
+    // step takes microseconds) ends the launch with an error;
+    // PYSIT_AMD_PERSIST_SPIN_LIMIT (>= 1) sets another bound, e.g. a tiny one
+    // that makes the time-out path run
+    long long spins = 1LL << 24;
+    if (const char *env = std::getenv("PYSIT_AMD_PERSIST_SPIN_LIMIT")) {
+        const long long v = std::atoll(env);
+        PA_CHECK(v >= 1, "PYSIT_AMD_PERSIST_SPIN_LIMIT must be an integer >= 1");
+        spins = v;
+    }
     return batch_run(b, mode, slot_begin, slot_end, direction, seg_base, active_tiles, active_recs, nullptr,
-                     stream, 1LL << 24);
+                     stream, spins);
 }
 
 int pysit_amd_batch_run_static(const pysit_amd_batch *b, int mode, int slot_begin, int slot_end,

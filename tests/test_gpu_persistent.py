@@ -92,3 +92,23 @@ def test_persistent_loop_reference_case(gpu):
         b.migrate([rng.standard_normal(t.shape) for t in tr])
         out.append((tr, b.ic.cpu().numpy().copy()))
     assert _same(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_persistent_loop_barrier_time_out_is_an_error(gpu, monkeypatch):
+    """A barrier wait beyond the spin limit ends every block and the call
+    reports it; the host forces it with a limit of one poll (every block's
+    wait is bounded, so the launch still ends normally).  The batch then
+    runs again at the default limit and gives the step loop's results."""
+    from pysit_amd import _native as N
+    probs = _uneven()
+    rng = np.random.default_rng(9)
+    res = [rng.standard_normal((p.nt, p.n_rec)) for p in probs]
+    ref = _gradient(_batch(probs, np.float64, 0), res)
+    b = _persistent(probs, np.float64)
+    monkeypatch.setenv("PYSIT_AMD_PERSIST_SPIN_LIMIT", "1")
+    print("persistent loop with a one-poll barrier limit", flush=True)
+    with pytest.raises(N.NativeError, match="timed out"):
+        b.forward(keep_dw=True)
+    monkeypatch.delenv("PYSIT_AMD_PERSIST_SPIN_LIMIT")
+    out = _gradient(b, res)
+    assert _same(out[0], ref[0]) and _same(out[1], ref[1])
