@@ -16,6 +16,8 @@ checkpointed = the same batched gradient under an 8 GiB memory budget: the
          forward keeps checkpoints, the adjoint recomputes each segment's
          dWaveOp; its gradient equals the store-all one bit for bit.
 plain  = the same gradient with one shot per launch (16 shot loops).
+c4_3d  = the 3D checkpointed gradient (--workload c4: 8 shots of 288^3,
+         nt 2000, FP32 order 8), one timed gradient in a child process.
 roofline: the adjoint kernel (the dominant one): algorithmic bytes per
          launch (FP32 fields: u_k, u_k-1, C read, u_k+1 written = 16 B per
          node; imaging: dWaveOp read 4 B + FP64 accumulator read/write 16 B;
@@ -51,6 +53,8 @@ def parse():
     p.add_argument("--no-plain", action="store_true", help="skip the one-shot-per-launch comparison")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-accuracy", action="store_true", help="skip the FP64 comparison gradient")
+    p.add_argument("--no-3d", action="store_true",
+                   help="c2: skip the 3D checkpointed gradient run in a child process after the 2D legs")
     p.add_argument("--cpu-seconds", type=float, default=6.0, help="target seconds per CPU baseline leg")
     p.add_argument("--tile-x", type=int, default=None)
     p.add_argument("--memory-budget-gb", type=float, default=None)
@@ -835,7 +839,8 @@ def main():
             "misfit_rel": abs(float(loss.detach()) - float(loss64.detach())) / float(loss64.detach()),
             "note": "FP32 path vs the FP64 device path on the same inputs; the reference's own FP32 step "
                     "arithmetic is 1.9e-4 off its FP64 traces on one of these shots (3000 steps)"}
-        del prob64
+        # the loss keeps its graph, whose node keeps the batch and its store
+        del prob64, loss64, m64, g64
         torch.cuda.empty_cache()
     if not (args.no_plain or args.profile_run):
         plain = [ShotProblem(solver, [s], memory_budget=budget, tile_x=args.tile_x) for s in shots]
@@ -858,7 +863,46 @@ def main():
         kp = solver.kernel_pml()
         result["cpu_baseline"] = cpu_baseline(shape, kp, dt, mesh.deltas, args.cpu_seconds)
         result["cpu_baseline"]["gradient"] = cpu_gradient(solver, shots[len(shots) // 2], args.cpu_seconds * 4)
+    if not (args.no_3d or args.profile_run):
+        # the 3D checkpointed gradient (--workload c4, one timed gradient) in
+        # a child process with the device memory to itself
+        import gc
+        # the last loss keeps its autograd graph, whose node keeps the
+        # batch and its dWaveOp store: release them before the child runs
+        problem.batch = None
+        loss = None
+        last.clear()
+        gc.collect()
+        torch.cuda.empty_cache()
+        free, total = torch.cuda.mem_get_info()
+        result["c4_3d"] = run_child_3d()
+        result["c4_3d"]["device_free_gib_at_start"] = free / 2 ** 30
     print(json.dumps(result), flush=True)
+
+
+def run_child_3d():
+    """--workload c4 --steps 1 --warmup 0 --no-cpu in a child process; its
+    JSON line, condensed (or the failure)."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--workload", "c4", "--steps", "1", "--warmup", "0",
+           "--no-cpu"]
+    print("[c2] 3D checkpointed gradient in a child process", file=sys.stderr, flush=True)
+    try:
+        p = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+    except subprocess.TimeoutExpired:
+        return {"error": "timed out after 900 s"}
+    lines = [l for l in p.stdout.splitlines() if l.startswith("{")]
+    if p.returncode != 0 or not lines:
+        return {"error": f"exit {p.returncode}", "stderr_tail": p.stderr[-800:]}
+    d = json.loads(lines[-1])
+    keep = ("metric", "value", "unit", "ms_per_step", "steps", "warmup", "gradient_roofline_frac",
+            "accuracy_vs_fp64", "config")
+    out = {k: d[k] for k in keep if k in d}
+    r = d.get("roofline", {})
+    out["roofline"] = {k: r.get(k) for k in ("kernel", "achieved", "peak", "unit", "frac", "traffic",
+                                              "avg_launch_us")}
+    out["command"] = " ".join(["python3", "bench.py"] + cmd[2:])
+    return out
 
 
 if __name__ == "__main__":
