@@ -109,11 +109,14 @@ def _faster(t, best_t):
     return t < best_t * (1.0 - MARGIN)
 
 
-def tuned_batch(shots, model, deltas, dt, order, dtype=np.float32, slots=32, report=None, **kw):
+def tuned_batch(shots, model, deltas, dt, order, dtype=np.float32, slots=2 * GRAPH_STEPS, report=None, **kw):
     """A :class:`DeviceBatch` of ``shots`` in the fastest launch
     configuration (keyword arguments other than the tuned ones pass through).
     ``report``, a list, receives (configuration, seconds per slot) of every
-    candidate timed."""
+    candidate timed.  Every candidate is timed over the same ``slots`` slots:
+    at least two graph chunks, so the graph candidate replays its graphs
+    (its first pass captures them) rather than falling back to single
+    launches."""
     for k in ("tile_x", "tile_order", "graph_steps"):
         kw.pop(k, None)
     sig = signature(shots, dtype, order, kw.get("imaging_period", 1), kw.get("ic_window"))

@@ -96,6 +96,26 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+_PLANS = {}
+
+
+def shared_plan(graph_steps, device=None):
+    """The graph plan of this process for a device and a graph step count,
+    shared by every batch there.  A graph is keyed by the whole batch
+    descriptor (every buffer address and size) and the run, so it replays
+    only with the kernel arguments it was captured with; a new batch whose
+    descriptor equals an earlier one's (per-shot batches get the same
+    buffers back from torch's allocator) replays the earlier graphs instead
+    of capturing its own.  Batches use it one at a time (one thread)."""
+    dev = torch.cuda.current_device() if device is None else int(device)
+    key = (dev, int(graph_steps))
+    plan = _PLANS.get(key)
+    if plan is None:
+        with torch.cuda.device(dev):
+            plan = _PLANS[key] = N.Plan(graph_steps, max_graphs=32)
+    return plan
+
+
 class ShotGeometry:
     """One propagation, in padded-grid terms.
 
@@ -807,7 +827,7 @@ class DeviceBatch:
                    _stream())
         elif self.graph_steps:
             if self.plan is None:
-                self.plan = N.Plan(self.graph_steps)
+                self.plan = shared_plan(self.graph_steps, self.dev.index)
             self.plan.run(b, mode, j0, j1, direction, seg_base, atp, arp, _stream())
         else:
             N.call("pysit_amd_batch_run", ctypes.byref(b), mode, j0, j1, direction, seg_base, atp, arp, _stream())
