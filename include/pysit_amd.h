@@ -36,7 +36,7 @@ const char *pysit_amd_version(void);
 /* Default tile geometry: rows along x (axis 0) or y (axis 1, 3D) of one
  * tile, for dim 2 or 3 and dtype 0 (f32) or 1 (f64).  A batch's tile_x /
  * tile_y carry these values or others pysit_amd_tile_rows_supported accepts
- * (2D: 8 or 4 rows along x; 3D: 16 x planes by 8 y rows). */
+ * (2D: 8, 4 or 2 rows along x; 3D: 16 x planes by 8 y rows). */
 int pysit_amd_tile_rows(int dim, int dtype, int axis);
 
 /* 1 if a batch of this dim and dtype may use tile_x rows along x, else 0

@@ -4,7 +4,7 @@ The step kernels of a batch run in several configurations that give the
 same results bit for bit and differ only in speed, which depends on the
 grids and the number of shots:
 
-* ``tile_x``: rows along x per wavefront tile (2D: 8 or 4; short tiles give
+* ``tile_x``: rows along x per wavefront tile (2D: 8, 4 or 2; short tiles give
   more wavefronts and shorter marches, which suits small batches);
 * ``tile_order``: the tile table order, ``'shot'`` or, for shots of one
   shape and length, ``'interleave'``;
@@ -35,6 +35,9 @@ _CACHE = {}
 # a candidate replaces the one before it (the default first) only when it is
 # this much faster: near ties keep the default, so repeated runs choose alike
 MARGIN = 0.03
+# device memory a probe's dWaveOp store may take (its slots are cut to fit,
+# down to 16); fewer slots than two graph chunks time single launches only
+PROBE_BYTES = 4 << 30
 
 
 def signature(shots, dtype, order, imaging_period=1, ic_window=None):
@@ -52,7 +55,7 @@ def candidates(shots, dtype):
     tried afterwards, for the fastest)."""
     dim = len(shots[0].shape)
     code = 1 if np.dtype(dtype) == np.float64 else 0
-    rows = [t for t in (N.tile_rows(dim, code, 0), 4, 8) if N.tile_rows_supported(dim, code, t)]
+    rows = [t for t in (N.tile_rows(dim, code, 0), 4, 2, 8) if N.tile_rows_supported(dim, code, t)]
     rows = list(dict.fromkeys(rows))
     alike = len({tuple(g.shape) for g in shots}) == 1 and len({g.nt for g in shots}) == 1
     # alike shots: the interleaved order first (the shots of a tile share
@@ -113,10 +116,12 @@ def tuned_batch(shots, model, deltas, dt, order, dtype=np.float32, slots=2 * GRA
     """A :class:`DeviceBatch` of ``shots`` in the fastest launch
     configuration (keyword arguments other than the tuned ones pass through).
     ``report``, a list, receives (configuration, seconds per slot) of every
-    candidate timed.  Every candidate is timed over the same ``slots`` slots:
-    at least two graph chunks, so the graph candidate replays its graphs
-    (its first pass captures them) rather than falling back to single
-    launches."""
+    candidate timed.  Every tile candidate is timed over the same ``slots``
+    slots (fewer where their dWaveOp store would pass PROBE_BYTES) with
+    single launches and with graphs: at least two graph chunks, so the graph
+    pass replays its graphs (its first pass captures them).
+    Small batches are bound by launch cost without graphs, which hides the
+    tile heights' difference; with graphs it shows."""
     for k in ("tile_x", "tile_order", "graph_steps"):
         kw.pop(k, None)
     sig = signature(shots, dtype, order, kw.get("imaging_period", 1), kw.get("ic_window"))
@@ -126,23 +131,15 @@ def tuned_batch(shots, model, deltas, dt, order, dtype=np.float32, slots=2 * GRA
         timed = []
         for cand in candidates(shots, dtype):
             b = DeviceBatch(shots, model, deltas, dt, order, dtype=dtype, graph_steps=0, **cand, **kw)
-            timed.append((b.probe_step_time(slots), dict(cand, graph_steps=0)))
+            S = min(int(slots), max(16, PROBE_BYTES // (b.total_n * b.dtype.itemsize)))
+            for gs in ((0, GRAPH_STEPS) if S >= 2 * GRAPH_STEPS else (0,)):
+                b.graph_steps = gs
+                timed.append((b.probe_step_time(S), dict(cand, graph_steps=gs)))
             del b
         best_t, best = timed[0]
         for t, c in timed[1:]:
             if _faster(t, best_t):
                 best_t, best = t, c
-        b = DeviceBatch(shots, model, deltas, dt, order, dtype=dtype,
-                        **dict(best, graph_steps=GRAPH_STEPS), **kw)
-        tg = b.probe_step_time(slots)
-        timed.append((tg, dict(best, graph_steps=GRAPH_STEPS)))
-        if _faster(tg, best_t):
-            cfg = dict(best, graph_steps=GRAPH_STEPS)
-            if report is not None:
-                report.extend((c, t) for t, c in timed)
-            _store(sig, cfg)
-            return b
-        del b
         torch.cuda.empty_cache()
         cfg = best
         if report is not None:

@@ -8,7 +8,7 @@
 //
 // Mapping to CDNA: one wavefront owns a tile of 64 consecutive z (the
 // contiguous axis: every row access is one coalesced 256 B transaction) by
-// TX = 8 or 4 rows along x.  The z tiles of a shot start at
+// TX = 8, 4 or 2 rows along x.  The z tiles of a shot start at
 // 0, 64, 128, ... except the last, which ends on the last node; the node
 // ranges the tiles own are disjoint (a tile lane that another tile owns only
 // loads), so the PML band at the far z side costs one PML tile, not two.
@@ -52,11 +52,12 @@
 
 namespace pa {
 
-// Tile rows along x: every 2D kernel is built for kTileRows2D[0] (the
-// default) and kTileRows2D[1] rows; a batch picks one (tile_x).  Short tiles
-// give more wavefronts and shorter marches (small grids), tall ones re-read
-// fewer halo rows.
-constexpr int kTileRows2D[2] = {8, 4};
+// Tile rows along x: every 2D step kernel is built for kTileRows2D[0] (the
+// default), [1] and [2] rows; a batch picks one (tile_x).  Short tiles give
+// more wavefronts and shorter marches (small grids: PySIT's 111 x 91
+// reflector case runs 8.1, 5.9 and 5.0 us per step at 8, 4 and 2 rows), tall
+// ones re-read fewer halo rows.  The persistent loop is built for [0] and [1].
+constexpr int kTileRows2D[3] = {8, 4, 2};
 
 // state buffers of one step: u_k, the other u buffer (batch: u_{k+1} is
 // written there; reference-contract step: it holds u_{k-1} and u_{k+1}
@@ -1021,13 +1022,14 @@ static int launch2d_r(int r, const pysit_amd_batch &B, const Step2DParams<T> &P,
     switch (B.tile_x) {
     case kTileRows2D[0]: return launch2d_tx<T, kTileRows2D[0], MODE>(r, B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
     case kTileRows2D[1]: return launch2d_tx<T, kTileRows2D[1], MODE>(r, B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
+    case kTileRows2D[2]: return launch2d_tx<T, kTileRows2D[2], MODE>(r, B, P, slot, seg_base, n_tiles, n_recs, st, cursor);
     }
     set_error("order must be 2, 4, 6 or 8");
     return PYSIT_AMD_EINVAL;
 }
 
 int tile_rows_2d(int) { return kTileRows2D[0]; }
-bool tile_rows_2d_supported(int tx) { return tx == kTileRows2D[0] || tx == kTileRows2D[1]; }
+bool tile_rows_2d_supported(int tx) { return tx == kTileRows2D[0] || tx == kTileRows2D[1] || tx == kTileRows2D[2]; }
 
 // ic += ic32; ic32 = 0 (the FP32 window accumulator's flush)
 __global__ void k_ic_flush(float *__restrict__ ic32, double *__restrict__ ic, long long n)
@@ -1080,8 +1082,9 @@ struct Sink2D final : StepSink {
 template <typename T>
 static int params2d(const pysit_amd_batch &B, Step2DParams<T> &P)
 {
-    PA_CHECK(B.tile_x == kTileRows2D[0] || B.tile_x == kTileRows2D[1],
-             "2D tiles are " + std::to_string(kTileRows2D[0]) + " or " + std::to_string(kTileRows2D[1]) +
+    PA_CHECK(tile_rows_2d_supported(B.tile_x),
+             "2D tiles are " + std::to_string(kTileRows2D[0]) + ", " + std::to_string(kTileRows2D[1]) + " or " +
+                 std::to_string(kTileRows2D[2]) +
                  " rows along x (tile_x)");
     P = Step2DParams<T>{};
     const int r = B.order / 2;
@@ -1151,6 +1154,9 @@ int run2d_persistent(const pysit_amd_batch &B, int mode, int slot_begin, int slo
     if (rc) return rc;
     PA_CHECK(mode == 0 || mode == 1 || mode == 2, "the persistent loop runs modes 0, 1 and 2");
     PA_CHECK(!B.dw_out && !B.vol_dw, "the persistent loop has no second-order terms");
+    PA_CHECK(B.tile_x == kTileRows2D[0] || B.tile_x == kTileRows2D[1],
+             "the persistent loop is built for " + std::to_string(kTileRows2D[0]) + "- and " +
+                 std::to_string(kTileRows2D[1]) + "-row tiles");
     const int n = slot_end - slot_begin;
     if (n <= 0) return PYSIT_AMD_OK;
     const bool window = mode == 2 && B.ic32 != nullptr;

@@ -284,7 +284,7 @@ class DeviceBatch:
         self.dt = float(dt)
         self.imaging_period = int(imaging_period)
         # tile rows along x: the library's default, or another height its
-        # kernels are built for (2D: 8 or 4)
+        # kernels are built for (2D: 8, 4 or 2)
         code = 1 if self.dtype == np.float64 else 0
         if tile_x:
             if not N.tile_rows_supported(self.dim, code, int(tile_x)):
@@ -325,9 +325,10 @@ class DeviceBatch:
         if graph_steps is not None and int(graph_steps) < 0:
             raise ValueError("graph_steps must be >= 0")
         self.plan = None
-        # 2D time loops without second-order terms as one persistent
-        # cooperative launch per run (pysit_amd_batch_run_persistent) in
-        # place of step launches or graphs.  The results are the same; its
+        # 2D time loops without second-order terms (8- or 4-row tiles) as one
+        # persistent cooperative launch per run
+        # (pysit_amd_batch_run_persistent) in place of step launches or
+        # graphs.  The results are the same; its
         # device-wide barrier between steps costs more than a graph's launch
         # boundary on MI355X, hence off by default.
         self.persistent = False
@@ -822,7 +823,8 @@ class DeviceBatch:
             ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
             ev[0].record()
         atp, arp = at.ctypes.data_as(ctypes.c_void_p), ar.ctypes.data_as(ctypes.c_void_p)
-        if self.persistent and self.dim == 2 and mode in (0, 1, 2) and not (b.dw_out or b.vol_dw):
+        if (self.persistent and self.dim == 2 and mode in (0, 1, 2) and not (b.dw_out or b.vol_dw)
+                and self.tile_x in (8, 4)):
             N.call("pysit_amd_batch_run_persistent", ctypes.byref(b), mode, j0, j1, direction, seg_base, atp, arp,
                    _stream())
         elif self.graph_steps:

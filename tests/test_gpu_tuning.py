@@ -17,7 +17,7 @@ def test_tile_heights_agree_bitwise(gpu, dtype):
     rng = np.random.default_rng(9)
     res = [rng.standard_normal((p.nt, p.n_rec)) for p in probs]
     outs = []
-    for tx in (8, 4):
+    for tx in (8, 4, 2):
         p0 = probs[0]
         b = DeviceBatch([p.geometry() for p in probs], p0.model, p0.deltas, p0.dt, p0.order, dtype=dtype,
                         tile_x=tx, graph_steps=0)
@@ -34,7 +34,7 @@ def test_tile_orders_and_heights_agree_for_alike_shots(gpu):
     rng = np.random.default_rng(2)
     res = [rng.standard_normal((p.nt, p.n_rec)) for p in probs]
     ref = None
-    for tx in (8, 4):
+    for tx in (8, 4, 2):
         for order in ("shot", "interleave"):
             p0 = probs[0]
             b = DeviceBatch([p.geometry() for p in probs], p0.model, p0.deltas, p0.dt, 8, dtype=np.float32,
