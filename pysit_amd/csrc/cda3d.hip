@@ -558,7 +558,10 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
     {
         auto yb = [&](int yy) { return yy < S.nyl || yy >= ny - S.nyr; };
         auto zb = [&](int zz) { return zz < S.nzl || zz >= nz - S.nzr; };
-        c.gen = P.single != 0;
+        // the batch steps (INC) never take the reference-contract step's
+        // general Phi input or dense rhs: compile-time false there, which
+        // frees their scalar registers
+        c.gen = !INC && P.single != 0;
         c.bp = yb(y) || zb(z);
         c.bph = yb(hy) || zb(z);
         c.bpz = yb(y) || zb(hz);
@@ -587,7 +590,7 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
     c.mx_off = S.mx_off;
     c.mx_lo = S.mx_lo;
     c.mx_hi = S.mx_hi;
-    c.has_rhs = P.rhs != nullptr;
+    c.has_rhs = !INC && P.rhs != nullptr;
     c.rhs = make_rsrc(c.has_rhs ? (const void *)P.rhs : vb, c.has_rhs ? nb : 0);
     c.shb = (int)(off * SZ);
     c.shb8 = (int)(off * 8);
