@@ -508,7 +508,7 @@ __device__ __forceinline__ void march3_full(const Ctx3D<T, R> &c, const Step3DPa
 #undef PA_ROW
 }
 
-template <typename T, int R, int MODE, bool INC>
+template <typename T, int R, int MODE, bool INC, bool FST>
 __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DParams<T> &P,
                                        const pysit_amd_shot &S, int tile, int full, int xclear, int psi, int tloc,
                                        int slot,
@@ -600,7 +600,9 @@ __device__ __forceinline__ void tile3d(const pysit_amd_batch &B, const Step3DPar
     const long long slot_bytes = B.total_n * SZ;
     const bool dwon = ((img || born3(MODE)) && B.dw) || store;
     c.dw = make_rsrc(dwon ? (const void *)((const T *)B.dw + slot_el) : vb, dwon ? slot_bytes : 0);
-    c.has_fst = B.fstore != nullptr;
+    // FST: the launch stores u_k of every step (B.fstore); kernels built
+    // without it hold no resource for that store
+    c.has_fst = FST && B.fstore != nullptr;
     c.fst = make_rsrc(c.has_fst ? (const void *)((const T *)B.fstore + slot_el) : vb, c.has_fst ? slot_bytes : 0);
     if constexpr (MODE == 4) c.ic = make_rsrc(B.ic32, B.total_n * 4);
     else c.ic = make_rsrc(ic64_3(MODE) ? (const void *)B.ic : vb, ic64_3(MODE) ? B.total_n * 8 : 0);
@@ -704,7 +706,7 @@ __device__ __forceinline__ void sample_group3(const pysit_amd_batch &B, int slot
 #endif
 // INC: the batch's increment form (du); false: the reference-contract step
 // (u_{k-1} in the second u buffer)
-template <typename T, int R, int MODE, bool INC>
+template <typename T, int R, int MODE, bool INC, bool FST>
 __global__ __launch_bounds__(Tile3<T>::TY * 64)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? PA3D_WAVES_PER_EU : 2)))
 void k_step3d(const pysit_amd_batch B, const Step3DParams<T> P,
@@ -729,12 +731,13 @@ void k_step3d(const pysit_amd_batch B, const Step3DParams<T> P,
     const int blk = xcd_block(b, tile_blocks);
     if (blk >= n_tile_blocks) return;
     if (!INC) {
-        tile3d<T, R, MODE, INC>(B, P, P.shot0, blk, 1, 0, 1, blk, slot, seg_base, L);
+        tile3d<T, R, MODE, INC, FST>(B, P, P.shot0, blk, 1, 0, 1, blk, slot, seg_base, L);
     } else {
         const int4 td = reinterpret_cast<const int4 *>(B.tiles)[blk];
         // flags bit2: the tile and its reach touch at most one PML band (no psi)
         const int psi = PA3D_PSI_SKIP ? ((td.z >> 2) & 1) ^ 1 : 1;
-        tile3d<T, R, MODE, INC>(B, P, B.shots[td.x], blk, td.z & 1, (td.z >> 1) & 1, psi, td.y, slot, seg_base, L);
+        tile3d<T, R, MODE, INC, FST>(B, P, B.shots[td.x], blk, td.z & 1, (td.z >> 1) & 1, psi, td.y, slot, seg_base,
+                                     L);
     }
 }
 
@@ -749,14 +752,17 @@ static int launch3d(const pysit_amd_batch &B, const Step3DParams<T> &P, int slot
     if (blocks == 0) return PYSIT_AMD_OK;
     if (P.single) {
         if constexpr (MODE == 0) {
-            hipLaunchKernelGGL((k_step3d<T, R, MODE, false>), dim3(blocks), dim3(TY3 * 64), 0, st, B, P, slot,
-                               seg_base, n_tiles, recs, rec_blocks, cursor);
+            hipLaunchKernelGGL((k_step3d<T, R, MODE, false, false>), dim3(blocks), dim3(TY3 * 64), 0, st, B, P,
+                               slot, seg_base, n_tiles, recs, rec_blocks, cursor);
         } else {
             set_error("the single-shot step only propagates");
             return PYSIT_AMD_EINVAL;
         }
+    } else if (B.fstore) {
+        hipLaunchKernelGGL((k_step3d<T, R, MODE, true, true>), dim3(blocks), dim3(TY3 * 64), 0, st, B, P, slot,
+                           seg_base, n_tiles, recs, rec_blocks, cursor);
     } else {
-        hipLaunchKernelGGL((k_step3d<T, R, MODE, true>), dim3(blocks), dim3(TY3 * 64), 0, st, B, P, slot,
+        hipLaunchKernelGGL((k_step3d<T, R, MODE, true, false>), dim3(blocks), dim3(TY3 * 64), 0, st, B, P, slot,
                            seg_base, n_tiles, recs, rec_blocks, cursor);
     }
     PA_HIP(hipGetLastError());
